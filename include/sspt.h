@@ -1,0 +1,151 @@
+/*
+ * sspt.h -- C ABI of libsspt.so, the MI355X (gfx950) pseudo-transient (PT)
+ * Stokes solver for a staggered spherical-sector grid.
+ *
+ * It replaces the ParallelStencil kernel calls of the reference scripts
+ * (PTsolvers/SphericalStokes) one for one, plus a fused PT loop.  Plain C:
+ * no C++ or torch types, no exceptions across the boundary.
+ *
+ * Ownership: the library owns all device buffers inside an opaque ss_ctx.
+ * Host arrays are caller-owned and are copied by ss_upload/ss_download.
+ * Every call is blocking (it returns after the GPU work is done), as each
+ * `@parallel` call of the reference is.  A context is bound to one device and
+ * is not thread-safe; use one context per host thread.
+ *
+ * Return codes: 0 = SS_OK; > 0 = a hipError_t from the runtime; < 0 = one of
+ * the SS_ERR_* codes below.  ss_last_error() returns a message.
+ *
+ * Field shapes (reference 1-based sizes, scripts_diapir/SphericalStokes_diapir.jl:381-399):
+ *   centre  (nr  ,nth  ,nph  )  P DT_R tau_RR tau_thth tau_phph eta eta_ini etaSM tII rho eta_PL
+ *   VR      (nr+1,nth  ,nph  )  VR VR_err
+ *   Vth     (nr  ,nth+1,nph  )  Vth Vth_err
+ *   Vph     (nr  ,nth  ,nph+1)  Vph Vph_err
+ *   tau_Rth (nr-1,nth-1,nph-2), tau_Rph (nr-1,nth-2,nph-1), tau_thph (nr-2,nth-1,nph-1)
+ */
+#ifndef SSPT_H
+#define SSPT_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ss_ctx ss_ctx;
+
+enum ss_status {
+    SS_OK = 0,
+    SS_ERR_ARG = -1,        /* bad size, null pointer, unknown id */
+    SS_ERR_NONFINITE = -2,  /* NaN in the convergence error (SphericalStokes_mxpu.jl:426) */
+    SS_NOT_CONVERGED = -3,  /* maxiter reached without err < eps */
+    SS_ERR_STATE = -4,      /* call order (e.g. no grid set) */
+    SS_ERR_NOMEM = -5       /* device allocation failed */
+};
+
+enum ss_field {
+    SS_P = 0, SS_DT_R, SS_TAU_RR, SS_TAU_THTH, SS_TAU_PHPH,
+    SS_TAU_RTH, SS_TAU_RPH, SS_TAU_THPH,
+    SS_VR, SS_VTH, SS_VPH, SS_VR_ERR, SS_VTH_ERR, SS_VPH_ERR,
+    SS_ETA, SS_ETA_INI, SS_ETASM, SS_TII, SS_RHO, SS_ETA_PL,
+    SS_NFIELDS
+};
+
+enum ss_layout {
+    SS_LAYOUT_JULIA = 0, /* column-major, r fastest: A[ix,iy,iz] of a Julia Array */
+    SS_LAYOUT_C = 1      /* row-major [ir][ith][iph], phi fastest */
+};
+
+enum ss_bc_mode {
+    SS_BC_SHEAR = 0,     /* diapir: simple shear on th/ph faces (SphericalStokes_diapir.jl:61-99) */
+    SS_BC_FREE_SLIP = 1, /* plateau: free slip on all faces (SphericalStokes_plateau.jl:64-92) */
+    SS_BC_PURE_SHEAR = 2 /* extension: pure shear, normal velocity prescribed on th/ph faces */
+};
+
+/* Scalars of one PT iteration, with the reference's parameter names. */
+typedef struct ss_params {
+    double _dr, _dth, _dph;  /* inverse grid spacings (SphericalStokes_diapir.jl:370) */
+    double dmp, vpdt, lr;    /* damping, PT step, radial extent (:367,400) */
+    double tau_C, n_exp;     /* power law (:355-356) */
+    double g;                /* gravity */
+    double r, sh;            /* sector centre radius and boundary shear rate (BCs) */
+    double relax;            /* 1.0 = reference power law; <1 log-space relaxation */
+    double eta_min, eta_max; /* viscosity cutoffs; 0 / +inf disable */
+    int bc_mode;             /* ss_bc_mode */
+    int err_drop_nan_vph;    /* plateau convergence test ignores a NaN Vph error */
+} ss_params;
+
+typedef struct ss_result {
+    int ittot;        /* reference's ittot (on break: iter-1, SphericalStokes_diapir.jl:439-441) */
+    int iter_end;     /* iteration at which err < eps && iter > 20, else 0 */
+    double err;       /* last evaluated err */
+    double err_comp[3];
+    double seconds;   /* wall time of iterations 11..ittot as the reference times it */
+} ss_result;
+
+/* ---- context, grid, data movement ---- */
+int ss_create(int nr, int nth, int nph, int device, ss_ctx **out);
+int ss_destroy(ss_ctx *ctx);
+const char *ss_last_error(const ss_ctx *ctx);
+/* Field dimensions (reference sizes) of a field id. */
+int ss_field_dims(const ss_ctx *ctx, int field, int dims[3]);
+/* 1D coordinates R_i[nr], th_i[nth], ph_i[nph] (SphericalStokes_diapir.jl:371-373);
+ * metric tables are built from them in FP64 on the host. */
+int ss_set_grid(ss_ctx *ctx, const double *R1, const double *th1, const double *ph1);
+int ss_upload(ss_ctx *ctx, int field, const double *host, int layout);
+int ss_download(ss_ctx *ctx, int field, double *host, int layout);
+/* A .= value over the field's reference shape (@zeros / @ones / rho0*@ones). */
+int ss_fill(ss_ctx *ctx, int field, double value);
+/* dst .= src, both of the same shape (eta_ini .= eta, :404). */
+int ss_copy_field(ss_ctx *ctx, int dst, int src);
+
+/* ---- setup kernels ---- */
+/* initialize_inclusion!(A,R,th,ph,r,radius,in), SphericalStokes_diapir.jl:127-137
+ * (compares the squared distance with `radius`, as the reference does). */
+int ss_initialize_inclusion(ss_ctx *ctx, int field, double r, double radius, double in);
+/* initialize_velocity!(Vph, th, fact, r), SphericalStokes_diapir.jl:139-146 */
+int ss_initialize_velocity(ss_ctx *ctx, double fact, double r);
+
+/* ---- per-kernel entry points (one per reference kernel) ---- */
+int ss_equal3(ss_ctx *ctx);                                   /* equal3!       :46-58   */
+int ss_maxloc(ss_ctx *ctx);                                   /* maxloc!       :149-152 */
+int ss_copy_bc(ss_ctx *ctx, int field);                       /* copy_BC!      :102-124 */
+int ss_compute_P(ss_ctx *ctx, double _dr, double _dth, double _dph,
+                 double dmp, double vpdt, double lr);         /* compute_P!    :209-241 */
+int ss_compute_TII(ss_ctx *ctx);                              /* compute_TII!  :243-257 */
+int ss_power_law(ss_ctx *ctx, double tau_C, double n_exp);    /* power_law!    :259-265 */
+/* power_law with log-space relaxation (CartesianStokes_diapir.jl:210-221) and cutoffs */
+int ss_power_law_ex(ss_ctx *ctx, double tau_C, double n_exp, double relax,
+                    double eta_min, double eta_max);
+int ss_compute_V(ss_ctx *ctx, double _dr, double _dth, double _dph, double g); /* compute_V! :308-329 */
+int ss_bc_R(ss_ctx *ctx);                                     /* BC_R!         :61-69   */
+int ss_bc_th(ss_ctx *ctx, int bc_mode, double r, double sh);  /* BC_th!        :71-86 / plateau :74-82 */
+int ss_bc_ph(ss_ctx *ctx, int bc_mode, double r, double sh);  /* BC_ph!        :88-99 / plateau :84-92 */
+/* check_err! (:332-344) + the host max-reductions of :428-432.
+ * err_comp[3] = max|V_err|/max|V| per component; *err = their max. */
+int ss_check_err(ss_ctx *ctx, int drop_nan_vph, double err_comp[3], double *err);
+
+/* ---- fused PT loop ---- */
+/* Runs `niter` PT iterations with the reference's loop semantics
+ * (SphericalStokes_diapir.jl:413-425) through the fused kernels.  All fields,
+ * including etaSM, DT_R and tII, equal what the per-kernel sequence leaves. */
+int ss_pt_iterate(ss_ctx *ctx, const ss_params *p, int niter);
+/* The PT loop with the reference exit test (err every nout, break when
+ * err < eps && iter > 20), SphericalStokes_diapir.jl:409-443.
+ * Returns SS_OK, SS_NOT_CONVERGED or SS_ERR_NONFINITE. */
+int ss_pt_solve(ss_ctx *ctx, const ss_params *p, int maxiter, int nout, double eps,
+                ss_result *out);
+/* Same loop through the per-kernel entry points (reference launch sequence). */
+int ss_pt_solve_unfused(ss_ctx *ctx, const ss_params *p, int maxiter, int nout,
+                        double eps, ss_result *out);
+
+/* ---- timing hooks for the benchmark (no reference counterpart) ---- */
+/* Average device time (ms) per iteration of the last ss_pt_iterate, from HIP
+ * events on the context's stream. */
+int ss_last_kernel_ms(const ss_ctx *ctx, double *ms_per_iter);
+/* ss_pt_iterate with HIP events around every launch of the r-marching main
+ * kernel: *main_ms = its average duration, *iter_ms = average time of a whole
+ * iteration (main kernel + boundary pass). */
+int ss_pt_iterate_timed(ss_ctx *ctx, const ss_params *p, int niter, double *main_ms, double *iter_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

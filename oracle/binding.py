@@ -1,0 +1,167 @@
+"""ctypes binding of the CPU oracle (oracle/libsspt_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_LIB = Path(__file__).resolve().parent / "libsspt_oracle.so"
+
+_FIELDS = ["P", "DT_R", "tRR", "tTT", "tPP", "tRT", "tRP", "tTP", "VR", "VT", "VP", "VR_err", "VT_err",
+           "VP_err", "eta", "eta_ini", "etaSM", "tII", "rho", "eta_PL"]
+# oracle struct member -> field name of sphericalstokes_amd.fields
+_NAME = {"tRR": "tau_RR", "tTT": "tau_thth", "tPP": "tau_phph", "tRT": "tau_Rth", "tRP": "tau_Rph",
+         "tTP": "tau_thph", "VT": "Vth", "VP": "Vph", "VT_err": "Vth_err", "VP_err": "Vph_err"}
+
+DP = C.POINTER(C.c_double)
+
+
+class orc_state(C.Structure):
+    _fields_ = [("nr", C.c_int), ("nth", C.c_int), ("nph", C.c_int)] + [(f, DP) for f in _FIELDS] + [
+        ("R1", DP), ("th1", DP), ("ph1", DP)]
+
+
+class orc_params(C.Structure):
+    _fields_ = [("_dr", C.c_double), ("_dth", C.c_double), ("_dph", C.c_double), ("dmp", C.c_double),
+                ("vpdt", C.c_double), ("lr", C.c_double), ("tau_C", C.c_double), ("n_exp", C.c_double),
+                ("g", C.c_double), ("r", C.c_double), ("sh", C.c_double), ("relax", C.c_double),
+                ("eta_min", C.c_double), ("eta_max", C.c_double), ("bc_mode", C.c_int),
+                ("err_drop_nan_vph", C.c_int)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not ORACLE_LIB.exists():
+            raise RuntimeError(f"{ORACLE_LIB} not built (make -C oracle)")
+        L = C.CDLL(str(ORACLE_LIB))
+        SP, PP = C.POINTER(orc_state), C.POINTER(orc_params)
+        I, D = C.c_int, C.c_double
+        sig = {
+            "orc_equal3": (None, [DP, DP, DP, DP, DP, DP, I, I, I]),
+            "orc_maxloc": (None, [DP, DP, I, I, I]),
+            "orc_copy_bc": (None, [DP, I, I, I]),
+            "orc_compute_P": (None, [SP, PP]),
+            "orc_compute_TII": (None, [SP]),
+            "orc_power_law": (None, [SP, PP]),
+            "orc_compute_V": (None, [SP, PP]),
+            "orc_bc_R": (None, [SP]),
+            "orc_bc_th": (None, [SP, PP]),
+            "orc_bc_ph": (None, [SP, PP]),
+            "orc_check_err": (D, [SP, PP, DP]),
+            "orc_initialize_inclusion": (None, [DP, DP, DP, DP, I, I, I, D, D, D]),
+            "orc_initialize_velocity": (None, [DP, DP, I, I, I, D, D]),
+            "orc_pt_step": (None, [SP, PP, I]),
+            "orc_pt_solve": (I, [SP, PP, I, I, D, I, DP, C.POINTER(I)]),
+            "orc_num_threads": (I, []),
+        }
+        for n, (res, args) in sig.items():
+            f = getattr(L, n)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(DP)
+
+
+def make_params(p) -> orc_params:
+    return orc_params(p._dr, p._dth, p._dph, p.dmp, p.vpdt, p.lr, p.tau_C, p.n_exp, p.g, p.r, p.sh, p.relax,
+                      p.eta_min, p.eta_max if p.eta_max is not None else math.inf, int(p.bc_mode),
+                      int(p.err_drop_nan_vph))
+
+
+class Oracle:
+    """Oracle state over numpy arrays in the reference layout (Fortran order).
+
+    ``fields`` maps sphericalstokes_amd field names to arrays; they are copied
+    (the oracle mutates its own copies in place)."""
+
+    def __init__(self, setup, fields=None, copy=True):
+        self.L = lib()
+        self.setup = setup
+        src = fields if fields is not None else setup.fields
+        if copy:
+            self.f = {k: np.array(v, dtype=np.float64, order="F", copy=True) for k, v in src.items()}
+        else:  # works on the caller's arrays in place (they must be Fortran-ordered float64)
+            self.f = dict(src)
+        self.R = np.ascontiguousarray(setup.R, dtype=np.float64)
+        self.th = np.ascontiguousarray(setup.th, dtype=np.float64)
+        self.ph = np.ascontiguousarray(setup.ph, dtype=np.float64)
+        s = orc_state()
+        s.nr, s.nth, s.nph = setup.nr, setup.nth, setup.nph
+        for m in _FIELDS:
+            a = self.f[_NAME.get(m, m)]
+            assert a.flags.f_contiguous
+            setattr(s, m, _ptr(a))
+        s.R1, s.th1, s.ph1 = _ptr(self.R), _ptr(self.th), _ptr(self.ph)
+        self.s = s
+        self.p = make_params(setup.params)
+
+    def set_params(self, p):
+        self.p = make_params(p)
+
+    # per-kernel calls
+    def maxloc(self):
+        s = self.s
+        self.L.orc_maxloc(s.etaSM, s.eta, s.nr, s.nth, s.nph)
+
+    def copy_bc(self, name):
+        a = self.f[name]
+        self.L.orc_copy_bc(_ptr(a), *a.shape)
+
+    def compute_P(self):
+        self.L.orc_compute_P(C.byref(self.s), C.byref(self.p))
+
+    def compute_TII(self):
+        self.L.orc_compute_TII(C.byref(self.s))
+
+    def power_law(self):
+        self.L.orc_power_law(C.byref(self.s), C.byref(self.p))
+
+    def compute_V(self):
+        self.L.orc_compute_V(C.byref(self.s), C.byref(self.p))
+
+    def bc_R(self):
+        self.L.orc_bc_R(C.byref(self.s))
+
+    def bc_th(self):
+        self.L.orc_bc_th(C.byref(self.s), C.byref(self.p))
+
+    def bc_ph(self):
+        self.L.orc_bc_ph(C.byref(self.s), C.byref(self.p))
+
+    def equal3(self):
+        s = self.s
+        self.L.orc_equal3(s.VR_err, s.VT_err, s.VP_err, s.VR, s.VT, s.VP, s.nr, s.nth, s.nph)
+
+    def check_err(self):
+        comp = (C.c_double * 3)()
+        e = self.L.orc_check_err(C.byref(self.s), C.byref(self.p), comp)
+        return e, tuple(comp)
+
+    def step(self, n=1, equal3=True):
+        for _ in range(n):
+            self.L.orc_pt_step(C.byref(self.s), C.byref(self.p), int(equal3))
+
+    def solve(self, maxiter, nout, eps, always_equal3=True):
+        err = C.c_double()
+        it_end = C.c_int()
+        ittot = self.L.orc_pt_solve(C.byref(self.s), C.byref(self.p), int(maxiter), int(nout), float(eps),
+                                    int(always_equal3), C.byref(err), C.byref(it_end))
+        return {"ittot": ittot, "iter_end": it_end.value, "err": err.value}
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

@@ -1,0 +1,99 @@
+"""ctypes binding of libsspt.so (the C ABI of include/sspt.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if it is missing or cannot be loaded, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libsspt.so"
+
+SS_OK, SS_ERR_ARG, SS_ERR_NONFINITE, SS_NOT_CONVERGED, SS_ERR_STATE, SS_ERR_NOMEM = 0, -1, -2, -3, -4, -5
+STATUS_NAMES = {0: "SS_OK", -1: "SS_ERR_ARG", -2: "SS_ERR_NONFINITE", -3: "SS_NOT_CONVERGED",
+                -4: "SS_ERR_STATE", -5: "SS_ERR_NOMEM"}
+
+
+class ss_params(C.Structure):
+    _fields_ = [
+        ("_dr", C.c_double), ("_dth", C.c_double), ("_dph", C.c_double),
+        ("dmp", C.c_double), ("vpdt", C.c_double), ("lr", C.c_double),
+        ("tau_C", C.c_double), ("n_exp", C.c_double), ("g", C.c_double),
+        ("r", C.c_double), ("sh", C.c_double), ("relax", C.c_double),
+        ("eta_min", C.c_double), ("eta_max", C.c_double),
+        ("bc_mode", C.c_int), ("err_drop_nan_vph", C.c_int),
+    ]
+
+    @classmethod
+    def from_params(cls, p) -> "ss_params":
+        return cls(p._dr, p._dth, p._dph, p.dmp, p.vpdt, p.lr, p.tau_C, p.n_exp, p.g, p.r, p.sh,
+                   p.relax, p.eta_min, p.eta_max if p.eta_max is not None else math.inf,
+                   int(p.bc_mode), int(p.err_drop_nan_vph))
+
+
+class ss_result(C.Structure):
+    _fields_ = [("ittot", C.c_int), ("iter_end", C.c_int), ("err", C.c_double),
+                ("err_comp", C.c_double * 3), ("seconds", C.c_double)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/sspt.h
+P = C.c_void_p
+D = C.c_double
+I = C.c_int
+DP = C.POINTER(C.c_double)
+SIGNATURES = {
+    "ss_create": (I, [I, I, I, I, C.POINTER(P)]),
+    "ss_destroy": (I, [P]),
+    "ss_last_error": (C.c_char_p, [P]),
+    "ss_field_dims": (I, [P, I, C.POINTER(C.c_int)]),
+    "ss_set_grid": (I, [P, DP, DP, DP]),
+    "ss_upload": (I, [P, I, DP, I]),
+    "ss_download": (I, [P, I, DP, I]),
+    "ss_fill": (I, [P, I, D]),
+    "ss_copy_field": (I, [P, I, I]),
+    "ss_initialize_inclusion": (I, [P, I, D, D, D]),
+    "ss_initialize_velocity": (I, [P, D, D]),
+    "ss_equal3": (I, [P]),
+    "ss_maxloc": (I, [P]),
+    "ss_copy_bc": (I, [P, I]),
+    "ss_compute_P": (I, [P, D, D, D, D, D, D]),
+    "ss_compute_TII": (I, [P]),
+    "ss_power_law": (I, [P, D, D]),
+    "ss_power_law_ex": (I, [P, D, D, D, D, D]),
+    "ss_compute_V": (I, [P, D, D, D, D]),
+    "ss_bc_R": (I, [P]),
+    "ss_bc_th": (I, [P, I, D, D]),
+    "ss_bc_ph": (I, [P, I, D, D]),
+    "ss_check_err": (I, [P, I, DP, DP]),
+    "ss_pt_iterate": (I, [P, C.POINTER(ss_params), I]),
+    "ss_pt_solve": (I, [P, C.POINTER(ss_params), I, I, D, C.POINTER(ss_result)]),
+    "ss_pt_solve_unfused": (I, [P, C.POINTER(ss_params), I, I, D, C.POINTER(ss_result)]),
+    "ss_last_kernel_ms": (I, [P, DP]),
+    "ss_pt_iterate_timed": (I, [P, C.POINTER(ss_params), I, DP, DP]),
+}
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """Loads libsspt.so once; raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class SSError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        self.code = code
+        name = STATUS_NAMES.get(code, f"hipError {code}" if code > 0 else str(code))
+        super().__init__(f"{name}: {msg}")

@@ -1,0 +1,229 @@
+// sspt_fused.hip -- one PT iteration in two launches: the r-marching fused
+// kernel (all interior updates) and the boundary pass (ghost faces of V and
+// boundary-cell viscosities).  The arithmetic lives in sspt_fused_body.h.
+#include "sspt_internal.h"
+#include "sspt_launch.h"
+#include "sspt_fused_body.h"
+#include <algorithm>
+#include <cstdlib>
+
+namespace sspt {
+
+using namespace fused;
+
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long v)
+{
+    for (int s = 32; s > 0; s >>= 1) {
+        const unsigned long long o = __shfl_xor(v, s, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void flush_max(const Args &a, const unsigned long long *mdv, const unsigned long long *mv,
+                                          int lane)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const unsigned long long d = wave_max(mdv[c]), v = wave_max(mv[c]);
+        if (lane == 0) {
+            if (d) atomicMax(&a.red[c], d);
+            if (v) atomicMax(&a.red[3 + c], v);
+        }
+    }
+}
+
+// Tiles are numbered (z, y, x) with x fastest.  Consecutive workgroups are
+// dealt round-robin to the 8 XCDs; the remap gives each XCD a contiguous run
+// of tiles so that th/phi neighbours (which re-read each other's edges) share
+// that XCD's L2.
+__device__ __forceinline__ void tile_of(int nx, int ny, int nz, int &bx, int &by, int &bz)
+{
+    const int T = nx * ny * nz;
+    const int per = (T + 7) / 8;
+    const int b = blockIdx.x;
+    const int t = (b % 8) * per + b / 8;
+    bx = t % nx;
+    by = (t / nx) % ny;
+    bz = t / (nx * ny);
+}
+
+__global__ void __launch_bounds__(TK * TJB) k_fused_main(Args a, int nx, int ny, int nz)
+{
+    __shared__ Lds L;
+    int bx, by, bz;
+    tile_of(nx, ny, nz, bx, by, bz);
+    if (bz >= nz) return; // padding slots of the remap (uniform per block)
+    const int kb = bx * OK_, jb = by * OJ;
+    const int s0 = bz * a.rchunk, s1 = min(s0 + a.rchunk, a.b.nr);
+    const int start = max(s0 - 2, -1);
+    Thread T;
+    thread_init(T, a, threadIdx.x, threadIdx.y, kb, jb);
+    thread_prologue(T, a, start);
+    for (int s = start; s < s1; ++s) {
+        const bool wr = s >= s0;
+        phase1(T, L, a, s);
+        __syncthreads();
+        phase2(T, L, a, s, wr);
+        __syncthreads();
+        phase3(T, L, a, s, wr);
+        __syncthreads();
+        // phase4 reads only STT/SPP/DT/TP/RT/RP; the next phase1 writes only
+        // VR/VT/VP/ETA, so no barrier is needed between them.
+        phase4(T, L, a, s, wr);
+    }
+    if (a.flags & F_CHECK) flush_max(a, T.mdv, T.mv, threadIdx.x);
+}
+
+// Boundary pass.  blockIdx.z = family: 0 -> r faces over (j,k),
+// 1 -> th faces over (i,k), 2 -> phi faces over (i,j).
+__global__ void __launch_bounds__(256) k_fused_boundary(Args a)
+{
+    const BoxG &b = a.b;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x; // fastest coordinate
+    const int v = blockIdx.y * blockDim.y + threadIdx.y;
+    const int fam = blockIdx.z;
+    unsigned long long mdv[3] = {0, 0, 0}, mv[3] = {0, 0, 0};
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    for (int c = 0; c < 3; ++c) {
+        const int n0 = c == 0 ? nr + 1 : nr, n1 = c == 1 ? nt + 1 : nt, n2 = c == 2 ? np + 1 : np;
+        if (fam == 0) {
+            if (u < n2 && v < n1) {
+                ghost_v_point(a, c, 0, v, u, &mdv[c], &mv[c]);
+                ghost_v_point(a, c, n0 - 1, v, u, &mdv[c], &mv[c]);
+            }
+        } else if (fam == 1) {
+            if (u < n2 && v < n0) {
+                ghost_v_point(a, c, v, 0, u, &mdv[c], &mv[c]);
+                ghost_v_point(a, c, v, n1 - 1, u, &mdv[c], &mv[c]);
+            }
+        } else {
+            if (u < n1 && v < n0) {
+                ghost_v_point(a, c, v, u, 0, &mdv[c], &mv[c]);
+                ghost_v_point(a, c, v, u, n2 - 1, &mdv[c], &mv[c]);
+            }
+        }
+    }
+    if (fam == 0) {
+        if (u < np && v < nt) {
+            boundary_eta_point(a, 0, v, u);
+            boundary_eta_point(a, nr - 1, v, u);
+        }
+    } else if (fam == 1) {
+        // each boundary cell exactly once: eta_PL is updated in place
+        if (u < np && v >= 1 && v < nr - 1) {
+            boundary_eta_point(a, v, 0, u);
+            boundary_eta_point(a, v, nt - 1, u);
+        }
+    } else {
+        if (u >= 1 && u < nt - 1 && v >= 1 && v < nr - 1) {
+            boundary_eta_point(a, v, u, 0);
+            boundary_eta_point(a, v, u, np - 1);
+        }
+    }
+    if (a.flags & F_CHECK) flush_max(a, mdv, mv, (threadIdx.x + threadIdx.y * blockDim.x) & 63);
+}
+
+__global__ void k_zero_red(unsigned long long *p)
+{
+    if (threadIdx.x < 6) p[threadIdx.x] = 0ull;
+}
+
+static Args make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out, const ss_params &p, bool last, bool check)
+{
+    static constexpr double PI = 3.14159265358979323846;
+    Args a;
+    a.b.nr = c->box.nr;
+    a.b.nt = c->box.nt;
+    a.b.np = c->box.np;
+    a.b.sj = c->box.sj;
+    a.b.si = c->box.si;
+    const Tables &t = c->tab;
+    a.t = TabsG{t.Rc, t.invRc, t.Rf, t.invRf, t.Re, t.invRe, t.Rc4, t.invRc4, t.Rc2, t.invRc2,
+                t.cotc, t.cotf4, t.cotc2, t.cotf2, t.rs_c, t.rs_e, t.rs_c4, t.rs_f, t.rs_y, t.rs_z, t.th, t.ph};
+    a.VR = in.f[SS_VR];
+    a.VT = in.f[SS_VTH];
+    a.VP = in.f[SS_VPH];
+    a.eta = in.f[SS_ETA];
+    a.P = in.f[SS_P];
+    a.tRR = in.f[SS_TAU_RR];
+    a.tTT = in.f[SS_TAU_THTH];
+    a.tPP = in.f[SS_TAU_PHPH];
+    a.tRT = in.f[SS_TAU_RTH];
+    a.tRP = in.f[SS_TAU_RPH];
+    a.tTP = in.f[SS_TAU_THPH];
+    a.rho = in.f[SS_RHO];
+    a.ini = in.f[SS_ETA_INI];
+    a.ePL = in.f[SS_ETA_PL];
+    a.oVR = out.f[SS_VR];
+    a.oVT = out.f[SS_VTH];
+    a.oVP = out.f[SS_VPH];
+    a.oEta = out.f[SS_ETA];
+    a.oP = out.f[SS_P];
+    a.oRR = out.f[SS_TAU_RR];
+    a.oTT = out.f[SS_TAU_THTH];
+    a.oPP = out.f[SS_TAU_PHPH];
+    a.oRT = out.f[SS_TAU_RTH];
+    a.oRP = out.f[SS_TAU_RPH];
+    a.oTP = out.f[SS_TAU_THPH];
+    a.oPL = out.f[SS_ETA_PL];
+    a.SM = out.f[SS_ETASM];
+    a.DT = out.f[SS_DT_R];
+    a.TII = out.f[SS_TII];
+    a.VRe = out.f[SS_VR_ERR];
+    a.VTe = out.f[SS_VTH_ERR];
+    a.VPe = out.f[SS_VPH_ERR];
+    a.red = c->red;
+    a._dr = p._dr;
+    a._dth = p._dth;
+    a._dph = p._dph;
+    a.kb = (((p.dmp * 2.0) * PI) * p.vpdt) / p.lr;
+    a.gs = ((4.0 * PI) * p.vpdt) / p.lr;
+    a.vpdt2 = p.vpdt * p.vpdt;
+    a.g = p.g;
+    a.tC = p.tau_C;
+    a.ex = 1.0 - p.n_exp;
+    a.relax = p.relax;
+    a.emin = p.eta_min;
+    a.emax = p.eta_max;
+    a.flags = (last ? F_LAST : 0) | (check ? F_CHECK : 0) | (p.relax != 1.0 ? F_RELAX : 0) |
+              (p.eta_min > 0.0 ? F_CLAMP_LO : 0) | (p.eta_max < INFINITY ? F_CLAMP_HI : 0);
+    a.bc_mode = p.bc_mode;
+    a.r = p.r;
+    a.sh = p.sh;
+    a.rchunk = c->box.nr;
+    return a;
+}
+
+// r-chunking: enough workgroups to fill 256 CUs several times over while
+// keeping the two warm-up planes of each chunk a small fraction.
+static int pick_rchunk(int nr, int tiles_xy)
+{
+    if (const char *e = std::getenv("SSPT_RCHUNK")) {
+        const int v = std::atoi(e);
+        if (v >= 1) return std::min(v, nr);
+    }
+    int nz = (4096 + tiles_xy - 1) / tiles_xy;
+    nz = std::max(1, std::min(nz, nr / 32));
+    return (nr + nz - 1) / nz;
+}
+
+void launch_fused_iteration(ss_ctx *c, const FieldSet &in, const FieldSet &out, const ss_params &p, bool last,
+                            bool check, hipEvent_t ev_main0, hipEvent_t ev_main1)
+{
+    Args a = make_args(c, in, out, p, last, check);
+    const int nx = (a.b.np + OK_ - 1) / OK_;
+    const int ny = (a.b.nt + OJ - 1) / OJ;
+    a.rchunk = pick_rchunk(a.b.nr, nx * ny);
+    const int nz = (a.b.nr + a.rchunk - 1) / a.rchunk;
+    if (check) hipLaunchKernelGGL(k_zero_red, dim3(1), dim3(64), 0, c->stream, c->red);
+    const int T = nx * ny * nz;
+    const int grid = ((T + 7) / 8) * 8;
+    if (ev_main0) (void)hipEventRecord(ev_main0, c->stream);
+    hipLaunchKernelGGL(k_fused_main, dim3(grid), dim3(TK, TJB), 0, c->stream, a, nx, ny, nz);
+    if (ev_main1) (void)hipEventRecord(ev_main1, c->stream);
+    const int m = std::max(std::max(a.b.nr, a.b.nt), a.b.np) + 1;
+    hipLaunchKernelGGL(k_fused_boundary, dim3((m + 63) / 64, (m + 3) / 4, 3), dim3(64, 4), 0, c->stream, a);
+}
+
+} // namespace sspt

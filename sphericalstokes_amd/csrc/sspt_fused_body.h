@@ -1,0 +1,599 @@
+// sspt_fused_body.h -- the fused PT iteration as a sequence of block phases.
+//
+// One launch of the main kernel performs a whole PT iteration
+// (SphericalStokes_diapir.jl:413-425: maxloc! + copy_BC!, compute_P!,
+// compute_TII! + copy_BC!, power_law!, compute_V!) for every interior point,
+// reading the old state and writing the new state into separate (ping-pong)
+// buffers.  A second, small kernel (boundary pass) then writes the ghost
+// faces of V (the BC_R!/BC_th!/BC_ph! sequence) and the boundary-cell
+// viscosities (power law of the copied tII).
+//
+// Block = 64 lanes along phi x 8 rows along th.  Lane tx covers k = kb-1+tx,
+// row ty covers j = jb-1+ty; the 62 x 6 inner points are owned (written),
+// lane 0 / row 0 are a low halo and lane 63 / row 7 a high halo that are
+// recomputed from the old state.  The block marches along r: step s handles
+// centre plane s, r-face s (VR) and r-face s+1 (tau_Rth, tau_Rph).  Values of
+// the current plane that neighbours need go through LDS; values the same
+// column needs at the next plane stay in registers (the Carry struct).
+//
+// The phases are plain functions of (thread state, LDS, args) separated by
+// block barriers, so the same code also runs in a host emulation that loops
+// over the threads of a block between barriers.
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+#ifndef SSPT_DH
+#define SSPT_DH __host__ __device__
+#endif
+
+namespace sspt {
+namespace fused {
+
+constexpr int TK = 64;        // lanes along phi (one wavefront)
+constexpr int TJB = 8;        // rows along th
+constexpr int OK_ = TK - 2;   // owned points per row
+constexpr int OJ = TJB - 2;   // owned rows
+constexpr double PI_ = 3.14159265358979323846;
+
+enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP_LO = 8, F_CLAMP_HI = 16 };
+
+struct BoxG {
+    int nr, nt, np;
+    long long sj, si;
+    SSPT_DH long long off(int i, int j, int k) const { return (long long)i * si + (long long)j * sj + k; }
+};
+
+struct TabsG {
+    const double *Rc, *invRc, *Rf, *invRf, *Re, *invRe, *Rc4, *invRc4, *Rc2, *invRc2;
+    const double *cotc, *cotf4, *cotc2, *cotf2;
+    const double *rs_c, *rs_e, *rs_c4, *rs_f, *rs_y, *rs_z;
+    const double *th, *ph;
+};
+
+struct Args {
+    BoxG b;
+    TabsG t;
+    // old state
+    const double *VR, *VT, *VP, *eta, *P, *tRR, *tTT, *tPP, *tRT, *tRP, *tTP, *rho, *ini, *ePL;
+    // new state
+    double *oVR, *oVT, *oVP, *oEta, *oP, *oRR, *oTT, *oPP, *oRT, *oRP, *oTP, *oPL;
+    // auxiliary outputs (last iteration / convergence check)
+    double *SM, *DT, *TII, *VRe, *VTe, *VPe;
+    unsigned long long *red; // [0..2] max|dV|, [3..5] max|V| as bit patterns
+    double _dr, _dth, _dph, kb, gs, vpdt2, g, tC, ex, relax, emin, emax;
+    int flags;
+    int bc_mode;
+    double r, sh;
+    int rchunk; // r-steps per block along grid z
+};
+
+struct Lds {
+    double VR[TJB][TK], VT[TJB][TK], VP[TJB][TK];
+    double ETA[TJB + 2][TK + 2];
+    double SM[TJB][TK];
+    double STT[TJB][TK], SPP[TJB][TK], DT[TJB][TK];
+    double RT[TJB][TK], RP[TJB][TK], TP[TJB][TK];
+};
+
+// Julia Base.max/min for Float64
+SSPT_DH inline double jmax(double x, double y)
+{
+    bool pick_y = (y > x) || (std::signbit(y) < std::signbit(x));
+    return pick_y ? (std::isnan(x) ? x : y) : (std::isnan(y) ? y : x);
+}
+SSPT_DH inline double jmin(double x, double y)
+{
+    bool pick_y = (y < x) || (std::signbit(y) > std::signbit(x));
+    return pick_y ? (std::isnan(x) ? x : y) : (std::isnan(y) ? y : x);
+}
+SSPT_DH inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+SSPT_DH inline unsigned long long absbits(double x)
+{
+    union { double d; unsigned long long u; } c;
+    c.d = std::fabs(x);
+    return c.u;
+}
+
+// power_law! (SphericalStokes_diapir.jl:259-265) with the optional log-space
+// relaxation (CartesianStokes_diapir.jl:210-221) and cutoffs.
+SSPT_DH inline double power_law_pt(double tii, double ini, double ePL_old, double *ePL_new, const Args &a)
+{
+    double pl = ini * std::pow(tii / a.tC, a.ex);
+    if (a.flags & F_RELAX) pl = std::exp(std::log(pl) * a.relax + std::log(ePL_old) * (1.0 - a.relax));
+    *ePL_new = pl;
+    double e = 2.0 / (1.0 / ini + 1.0 / pl);
+    if (a.flags & F_CLAMP_LO) e = jmax(e, a.emin);
+    if (a.flags & F_CLAMP_HI) e = jmin(e, a.emax);
+    return e;
+}
+
+struct Thread {
+    int tx, ty, j, k;
+    int jc, kc;       // j,k clamped into the box for loads
+    bool own;         // owned (written) position
+    // registers carried along r
+    double vr0, vr1, vt0, vt1, vp0, vp1;     // old V at planes s, s+1 (own column)
+    double e0, e1, e2;                       // old eta at s, s+1, s+2
+    double sm0, sm1;                         // etaSM at s, s+1
+    double cRR, cTT, cPP, cDT, crho;         // sigma/DT/rho of plane s-1 (VR face s)
+    double rt0, rt0n, rp0, rp0n;             // new tau_Rth/tau_Rph of face s (own, +1 neighbour)
+    double pG_rt, pE_rt, pG_rp, pE_rp;       // partial 4-point sums of plane s for face s+1
+    double gtp, etp;                         // 4-point G / eta averages for tau_thph(s)
+    // values of the current step
+    double oP, oRR, oTT, oPP, oRTold, oRPold, oTPold, orho, oini, oPL;
+    double nRR, nTT, nPP, nSRR, nSTT, nSPP, nDT;
+    double nrt, nrp, ntp;
+    double pG_rt2, pE_rt2, pG_rp2, pE_rp2, gtp2, etp2;
+    unsigned long long mdv[3], mv[3];
+};
+
+SSPT_DH inline void thread_init(Thread &T, const Args &a, int tx, int ty, int kb, int jb)
+{
+    T.tx = tx;
+    T.ty = ty;
+    T.k = kb - 1 + tx;
+    T.j = jb - 1 + ty;
+    T.kc = clampi(T.k, 0, a.b.np);
+    T.jc = clampi(T.j, 0, a.b.nt);
+    T.own = tx >= 1 && tx <= TK - 2 && ty >= 1 && ty <= TJB - 2 && T.j < a.b.nt && T.k < a.b.np;
+    for (int c = 0; c < 3; ++c) T.mdv[c] = T.mv[c] = 0ull;
+    T.vr0 = T.vr1 = T.vt0 = T.vt1 = T.vp0 = T.vp1 = 0.0;
+    T.e0 = T.e1 = T.e2 = 1.0;
+    T.sm0 = T.sm1 = 1.0;
+    T.cRR = T.cTT = T.cPP = T.cDT = T.crho = 0.0;
+    T.rt0 = T.rt0n = T.rp0 = T.rp0n = 0.0;
+    T.pG_rt = T.pE_rt = T.pG_rp = T.pE_rp = T.gtp = T.etp = 1.0;
+    T.nrt = T.nrp = T.ntp = 0.0;
+}
+
+// Loads of the first plane(s) before the march (start = first step).
+SSPT_DH inline void thread_prologue(Thread &T, const Args &a, int start)
+{
+    const BoxG &b = a.b;
+    const int i0 = clampi(start, 0, b.nr - 1), i1 = clampi(start + 1, 0, b.nr - 1);
+    T.e0 = a.eta[b.off(i0, clampi(T.jc, 0, b.nt - 1), clampi(T.kc, 0, b.np - 1))];
+    T.e1 = a.eta[b.off(i1, clampi(T.jc, 0, b.nt - 1), clampi(T.kc, 0, b.np - 1))];
+    const int iv = clampi(start, 0, b.nr);
+    T.vr1 = a.VR[b.off(iv, T.jc, T.kc)];
+    T.vt1 = a.VT[b.off(clampi(start, 0, b.nr - 1), T.jc, T.kc)];
+    T.vp1 = a.VP[b.off(clampi(start, 0, b.nr - 1), T.jc, T.kc)];
+}
+
+// ---- phase 1: stage old V and eta of the coming planes, issue own loads
+SSPT_DH inline void phase1(Thread &T, Lds &L, const Args &a, int s)
+{
+    const BoxG &b = a.b;
+    const int nr = b.nr;
+    // shift the r window: values loaded for plane s+1 at step s-1 become plane s
+    T.vr0 = T.vr1;
+    T.vt0 = T.vt1;
+    T.vp0 = T.vp1;
+    T.vr1 = a.VR[b.off(clampi(s + 1, 0, nr), T.jc, T.kc)];
+    T.vt1 = a.VT[b.off(clampi(s + 1, 0, nr - 1), T.jc, T.kc)];
+    T.vp1 = a.VP[b.off(clampi(s + 1, 0, nr - 1), T.jc, T.kc)];
+    const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
+    T.e2 = a.eta[b.off(clampi(s + 2, 0, nr - 1), jcc, kcc)];
+    L.VR[T.ty][T.tx] = T.vr1;
+    L.VT[T.ty][T.tx] = T.vt0;
+    L.VP[T.ty][T.tx] = T.vp0;
+    // eta plane s+1 with a one-point ring for the 7-point star of the halo
+    const int ie = clampi(s + 1, 0, nr - 1);
+    L.ETA[T.ty + 1][T.tx + 1] = T.e1;
+    if (T.ty == 0) L.ETA[0][T.tx + 1] = a.eta[b.off(ie, clampi(T.j - 1, 0, b.nt - 1), kcc)];
+    if (T.ty == TJB - 1) L.ETA[TJB + 1][T.tx + 1] = a.eta[b.off(ie, clampi(T.j + 1, 0, b.nt - 1), kcc)];
+    if (T.tx == 0) L.ETA[T.ty + 1][0] = a.eta[b.off(ie, jcc, clampi(T.k - 1, 0, b.np - 1))];
+    if (T.tx == TK - 1) L.ETA[T.ty + 1][TK + 1] = a.eta[b.off(ie, jcc, clampi(T.k + 1, 0, b.np - 1))];
+    // own old values used by this step
+    const int ic = clampi(s, 0, nr - 1);
+    const long long oc = b.off(ic, jcc, kcc);
+    T.oP = a.P[oc];
+    T.oRR = a.tRR[oc];
+    T.oTT = a.tTT[oc];
+    T.oPP = a.tPP[oc];
+    T.orho = a.rho[oc];
+    T.oini = a.ini[oc];
+    T.oPL = (a.flags & F_RELAX) ? a.ePL[oc] : 0.0;
+    T.oTPold = a.tTP[b.off(ic, T.jc, T.kc)];
+    const int ie1 = clampi(s + 1, 0, nr);
+    T.oRTold = a.tRT[b.off(ie1, T.jc, T.kc)];
+    T.oRPold = a.tRP[b.off(ie1, T.jc, T.kc)];
+}
+
+// 7-point star max of eta at (p, cj, ck) with in-plane values from LDS
+// (@maxloc expansion, SphericalStokes_mxpu.jl:295-297).
+SSPT_DH inline double star(const Lds &L, int ty, int tx, double em, double ep)
+{
+    const double c = L.ETA[ty + 1][tx + 1];
+    return jmax(jmax(jmax(jmax(em, ep), c), jmax(L.ETA[ty][tx + 1], L.ETA[ty + 2][tx + 1])),
+                jmax(L.ETA[ty + 1][tx], L.ETA[ty + 1][tx + 2]));
+}
+
+// ---- phase 2: etaSM(s+1); C(s) = P, tau_nn, DT at centres; tau_thph(s); VR face s
+SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
+{
+    const BoxG &b = a.b;
+    const TabsG &t = a.t;
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    const int p = s + 1;
+    // etaSM of plane p after copy_BC!: star at the clamped position
+    if (p >= 1 && p <= nr - 2) {
+        const int cj = clampi(T.j, 1, nt - 2), ck = clampi(T.k, 1, np - 2);
+        if (cj == T.j && ck == T.k) {
+            T.sm1 = star(L, T.ty, T.tx, T.e0, T.e2);
+        } else {
+            const int dty = cj - T.j, dtx = ck - T.k;
+            const int ty = clampi(T.ty + dty, 0, TJB - 1), tx = clampi(T.tx + dtx, 0, TK - 1);
+            const double em = a.eta[b.off(p - 1, cj, ck)], ep = a.eta[b.off(p + 1, cj, ck)];
+            T.sm1 = star(L, ty, tx, em, ep);
+        }
+    } else if (p == nr - 1) {
+        T.sm1 = T.sm0;
+    }
+    if (s == 0) T.sm0 = T.sm1; // etaSM(0) = etaSM(1)
+    L.SM[T.ty][T.tx] = T.sm1;
+
+    const bool centre = T.j >= 0 && T.j < nt && T.k >= 0 && T.k < np;
+    // C(s): compute_P! centre part
+    if (s >= 0 && s <= nr - 1 && centre && T.tx < TK - 1 && T.ty < TJB - 1) {
+        const double R = t.Rc[s];
+        const double dVRr = (T.vr1 - T.vr0) * a._dr;
+        const double dVTt = (t.invRc[s] * (L.VT[T.ty + 1][T.tx] - T.vt0)) * a._dth;
+        const double dVPp = (t.rs_c[s * (nt + 1) + T.j] * (L.VP[T.ty][T.tx + 1] - T.vp0)) * a._dph;
+        const double avVR = (T.vr0 + T.vr1) * 0.5;
+        const double avVT = (T.vt0 + L.VT[T.ty + 1][T.tx]) * 0.5;
+        const double ct = t.cotc[T.j];
+        const double vr_R = avVR / R, vt_R = avVT / R;
+        const double divV = (((dVRr + dVTt) + dVPp) + (2.0 * avVR) / R) + vt_R * ct;
+        const double third = 1.0 / 3.0;
+        const double Drr = dVRr - third * divV;
+        const double Dtt = (dVTt + vr_R) - third * divV;
+        const double Dpp = ((dVPp + vr_R) + vt_R * ct) - third * divV;
+        const double KB = a.kb * T.sm0, GS = a.gs * T.sm0;
+        const double den = 1.0 + GS / T.e0;
+        const double nP = T.oP - KB * divV;
+        T.nRR = (T.oRR + (GS * 2.0) * Drr) / den;
+        T.nTT = (T.oTT + (GS * 2.0) * Dtt) / den;
+        T.nPP = (T.oPP + (GS * 2.0) * Dpp) / den;
+        T.nDT = a.vpdt2 / (KB + GS / den);
+        T.nSRR = -nP + T.nRR;
+        T.nSTT = -nP + T.nTT;
+        T.nSPP = -nP + T.nPP;
+        if (wr && T.own) {
+            const long long o = b.off(s, T.j, T.k);
+            a.oP[o] = nP;
+            a.oRR[o] = T.nRR;
+            a.oTT[o] = T.nTT;
+            a.oPP[o] = T.nPP;
+            if (a.flags & F_LAST) {
+                a.DT[o] = T.nDT;
+                a.SM[o] = T.sm0;
+            }
+        }
+    } else {
+        T.nRR = T.nTT = T.nPP = 0.0;
+        T.nSRR = T.nSTT = T.nSPP = 0.0;
+        T.nDT = 0.0;
+    }
+    L.STT[T.ty][T.tx] = T.nSTT;
+    L.SPP[T.ty][T.tx] = T.nSPP;
+    L.DT[T.ty][T.tx] = T.nDT;
+
+    // tau_thph(s): compute_P! edge part, plane s, th-face j, phi-face k
+    T.ntp = 0.0;
+    if (s >= 1 && s <= nr - 2 && T.tx >= 1 && T.ty >= 1 && T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 1) {
+        const double dVP = T.vp0 - L.VP[T.ty - 1][T.tx];
+        const double dVT = T.vt0 - L.VT[T.ty][T.tx - 1];
+        const double avVP = (L.VP[T.ty - 1][T.tx] + T.vp0) * 0.5;
+        const double D = ((t.invRc4[s] * dVP) * a._dth + (t.rs_c4[s * (nt + 1) + T.j] * dVT) * a._dph) -
+                         (avVP / t.Rc4[s]) * t.cotf4[T.j];
+        T.ntp = (T.oTPold + T.gtp * D) / (1.0 + T.gtp / T.etp);
+        if (wr && T.own) a.oTP[b.off(s, T.j, T.k)] = T.ntp;
+    }
+    L.TP[T.ty][T.tx] = T.ntp;
+
+    // VR at r-face s (compute_V!): needs C(s-1), C(s), tau_Rth/tau_Rph of face s
+    if (wr && T.own && s >= 1 && s <= nr - 1 && T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 2) {
+        const double Rx = t.Rf[s];
+        const double sRR = (T.cRR + T.nSRR) * 0.5;
+        const double sTT = (T.cTT + T.nSTT) * 0.5;
+        const double sPP = (T.cPP + T.nSPP) * 0.5;
+        const double aRT = (T.rt0 + T.rt0n) * 0.5;
+        const double rg = (T.crho * a.g + T.orho * a.g) * 0.5;
+        const double dV = ((((((T.nSRR - T.cRR) * a._dr + (t.invRf[s] * (T.rt0n - T.rt0)) * a._dth) +
+                              (t.rs_f[s * (nt + 1) + T.j] * (T.rp0n - T.rp0)) * a._dph) +
+                             (2.0 * sRR) / Rx) -
+                            (sTT + sPP) / Rx) +
+                           (aRT / Rx) * t.cotc2[T.j]) -
+                          rg;
+        const double dt = (T.cDT + T.nDT) * 0.5;
+        const double v = T.vr0 + dV * dt;
+        const long long o = b.off(s, T.j, T.k);
+        a.oVR[o] = v;
+        if (a.flags & F_CHECK) {
+            const double d = v - T.vr0;
+            a.VRe[o] = d;
+            const unsigned long long ud = absbits(d), uv = absbits(v);
+            T.mdv[0] = ud > T.mdv[0] ? ud : T.mdv[0];
+            T.mv[0] = uv > T.mv[0] ? uv : T.mv[0];
+        } else if (a.flags & F_LAST) {
+            a.VRe[o] = T.vr0;
+        }
+    }
+}
+
+// ---- phase 3: tau_Rth / tau_Rph on r-face s+1; 4-point sums of plane s+1
+SSPT_DH inline void phase3(Thread &T, Lds &L, const Args &a, int s, bool wr)
+{
+    const BoxG &b = a.b;
+    const TabsG &t = a.t;
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    const int f = s + 1;
+    const double gs = a.gs;
+    const bool inner = T.tx >= 1 && T.ty >= 1;
+    // sums over plane s+1 (etaSM(s+1) in L.SM, eta(s+1) in L.ETA)
+    if (inner) {
+        const double smJ = L.SM[T.ty - 1][T.tx], smK = L.SM[T.ty][T.tx - 1], smJK = L.SM[T.ty - 1][T.tx - 1];
+        const double eJ = L.ETA[T.ty][T.tx + 1], eK = L.ETA[T.ty + 1][T.tx], eJK = L.ETA[T.ty][T.tx];
+        T.pG_rt2 = gs * smJ + gs * T.sm1;
+        T.pE_rt2 = eJ + T.e1;
+        T.pG_rp2 = gs * smK + gs * T.sm1;
+        T.pE_rp2 = eK + T.e1;
+        T.gtp2 = (((gs * smJK + gs * smJ) + gs * smK) + gs * T.sm1) * 0.25;
+        T.etp2 = (((eJK + eJ) + eK) + T.e1) * 0.25;
+        if (s == 0) { // etaSM(0,:,:) = etaSM(1,:,:): plane-0 sums of gs*etaSM are those of plane 1
+            T.pG_rt = T.pG_rt2;
+            T.pG_rp = T.pG_rp2;
+        }
+    }
+    T.nrt = 0.0;
+    T.nrp = 0.0;
+    if (f >= 1 && f <= nr - 1 && inner) {
+        const double Re = t.Re[f];
+        if (T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 2) {
+            const double G = ((T.pG_rt + gs * L.SM[T.ty - 1][T.tx]) + gs * T.sm1) * 0.25;
+            const double E = ((T.pE_rt + L.ETA[T.ty][T.tx + 1]) + T.e1) * 0.25;
+            const double D = ((T.vt1 - T.vt0) * a._dr + (t.invRe[f] * (T.vr1 - L.VR[T.ty - 1][T.tx])) * a._dth) -
+                             ((T.vt0 + T.vt1) * 0.5) / Re;
+            T.nrt = (T.oRTold + G * D) / (1.0 + G / E);
+            if (wr && T.own) a.oRT[b.off(f, T.j, T.k)] = T.nrt;
+        }
+        if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
+            const double G = ((T.pG_rp + gs * L.SM[T.ty][T.tx - 1]) + gs * T.sm1) * 0.25;
+            const double E = ((T.pE_rp + L.ETA[T.ty + 1][T.tx]) + T.e1) * 0.25;
+            const double D = ((T.vp1 - T.vp0) * a._dr + (t.rs_e[f * (nt + 1) + T.j] * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
+                             ((T.vp0 + T.vp1) * 0.5) / Re;
+            T.nrp = (T.oRPold + G * D) / (1.0 + G / E);
+            if (wr && T.own) a.oRP[b.off(f, T.j, T.k)] = T.nrp;
+        }
+    }
+    L.RT[T.ty][T.tx] = T.nrt;
+    L.RP[T.ty][T.tx] = T.nrp;
+}
+
+// ---- phase 4: Vth(s), Vph(s), tII(s) -> eta(s); shift carries
+SSPT_DH inline void phase4(Thread &T, Lds &L, const Args &a, int s, bool wr)
+{
+    const BoxG &b = a.b;
+    const TabsG &t = a.t;
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    if (wr && T.own && s >= 1 && s <= nr - 2) {
+        const double Ry = t.Rc2[s];
+        // Vth at (s, j, k)
+        if (T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 2) {
+            const double sTTm = L.STT[T.ty - 1][T.tx], sPPm = L.SPP[T.ty - 1][T.tx];
+            const double sTT = (sTTm + T.nSTT) * 0.5, sPP = (sPPm + T.nSPP) * 0.5;
+            const double aRT = (T.rt0 + T.nrt) * 0.5;
+            const double tpK = L.TP[T.ty][T.tx + 1];
+            const double dV = ((((T.nrt - T.rt0) * a._dr + (t.invRc2[s] * (T.nSTT - sTTm)) * a._dth) +
+                                (t.rs_y[s * (nt + 1) + T.j] * (tpK - T.ntp)) * a._dph) +
+                               (3.0 * aRT) / Ry) +
+                              ((sTT - sPP) / Ry) * t.cotf2[T.j];
+            const double dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * 0.5;
+            const double v = T.vt0 + dV * dt;
+            const long long o = b.off(s, T.j, T.k);
+            a.oVT[o] = v;
+            if (a.flags & F_CHECK) {
+                const double d = v - T.vt0;
+                a.VTe[o] = d;
+                const unsigned long long ud = absbits(d), uv = absbits(v);
+                T.mdv[1] = ud > T.mdv[1] ? ud : T.mdv[1];
+                T.mv[1] = uv > T.mv[1] ? uv : T.mv[1];
+            } else if (a.flags & F_LAST) {
+                a.VTe[o] = T.vt0;
+            }
+        }
+        // Vph at (s, j, k)
+        if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
+            const double tpJ = L.TP[T.ty + 1][T.tx];
+            const double sPPm = L.SPP[T.ty][T.tx - 1];
+            const double aRP = (T.rp0 + T.nrp) * 0.5;
+            const double aTP = (T.ntp + tpJ) * 0.5;
+            const double dV = ((((T.nrp - T.rp0) * a._dr + (t.invRc2[s] * (tpJ - T.ntp)) * a._dth) +
+                                (t.rs_z[s * (nt + 1) + T.j] * (T.nSPP - sPPm)) * a._dph) +
+                               (3.0 * aRP) / Ry) +
+                              ((2.0 * aTP) / Ry) * t.cotc2[T.j];
+            const double dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * 0.5;
+            const double v = T.vp0 + dV * dt;
+            const long long o = b.off(s, T.j, T.k);
+            a.oVP[o] = v;
+            if (a.flags & F_CHECK) {
+                const double d = v - T.vp0;
+                a.VPe[o] = d;
+                const unsigned long long ud = absbits(d), uv = absbits(v);
+                T.mdv[2] = ud > T.mdv[2] ? ud : T.mdv[2];
+                T.mv[2] = uv > T.mv[2] ? uv : T.mv[2];
+            } else if (a.flags & F_LAST) {
+                a.VPe[o] = T.vp0;
+            }
+        }
+        // tII and the new viscosity of interior cells
+        if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 2) {
+            const double x = (((T.rt0 + T.rt0n) + T.nrt) + L.RT[T.ty + 1][T.tx]) * 0.25;
+            const double y = (((T.rp0 + T.rp0n) + T.nrp) + L.RP[T.ty][T.tx + 1]) * 0.25;
+            const double z = (((T.ntp + L.TP[T.ty][T.tx + 1]) + L.TP[T.ty + 1][T.tx]) + L.TP[T.ty + 1][T.tx + 1]) * 0.25;
+            const double tii =
+                std::sqrt((((1.0 / 2.0) * ((T.nRR * T.nRR + T.nTT * T.nTT) + T.nPP * T.nPP) + x * x) + y * y) + z * z);
+            double pl;
+            const double e = power_law_pt(tii, T.oini, T.oPL, &pl, a);
+            const long long o = b.off(s, T.j, T.k);
+            a.oEta[o] = e;
+            if (a.flags & F_RELAX) a.oPL[o] = pl;
+            if (a.flags & F_LAST) a.TII[o] = tii;
+        }
+    }
+    // carries for step s+1
+    if (T.ty + 1 < TJB) T.rt0n = L.RT[T.ty + 1][T.tx];
+    if (T.tx + 1 < TK) T.rp0n = L.RP[T.ty][T.tx + 1];
+    T.rt0 = T.nrt;
+    T.rp0 = T.nrp;
+    T.cRR = T.nSRR;
+    T.cTT = T.nSTT;
+    T.cPP = T.nSPP;
+    T.cDT = T.nDT;
+    T.crho = T.orho;
+    T.e0 = T.e1;
+    T.e1 = T.e2;
+    T.sm0 = T.sm1;
+    T.pG_rt = T.pG_rt2;
+    T.pE_rt = T.pE_rt2;
+    T.pG_rp = T.pG_rp2;
+    T.pE_rp = T.pE_rp2;
+    T.gtp = T.gtp2;
+    T.etp = T.etp2;
+}
+
+// ---------------------------------------------------------------- boundary pass
+// Final value of a velocity component at a ghost position after the
+// BC_R! -> BC_th! -> BC_ph! sequence: the last BC among the directions in
+// which the position is a ghost decides it, from the final value at the
+// inward neighbour (which is a ghost only in earlier directions).
+// comp: 0 = VR, 1 = Vth, 2 = Vph.
+SSPT_DH inline bool ghost_dir(const BoxG &b, int comp, int dir, int idx, int *inward)
+{
+    int n;
+    if (dir == 0) n = (comp == 0) ? b.nr + 1 : b.nr;
+    else if (dir == 1) n = (comp == 1) ? b.nt + 1 : b.nt;
+    else n = (comp == 2) ? b.np + 1 : b.np;
+    if (idx == 0) { *inward = 1; return true; }
+    if (idx == n - 1) { *inward = n - 2; return true; }
+    return false;
+}
+
+// affine BC map: value = a*v + c  (a in {-1,0,1})
+SSPT_DH inline double bc_apply(const Args &A, int comp, int dir, int side, int j, double v)
+{
+    const BoxG &b = A.b;
+    const int mode = A.bc_mode; // 0 shear, 1 free slip, 2 pure shear
+    if (dir == 0) return comp == 0 ? -v : v;
+    if (dir == 1) {
+        if (comp == 1) {
+            if (mode == 2) {
+                const double th = side ? A.t.th[b.nt - 1] : A.t.th[0];
+                const double vb = -(((th - PI_ / 2.0) * A.r) * A.sh);
+                return 2.0 * vb - v;
+            }
+            return -v;
+        }
+        if (mode == 0) {
+            if (comp == 0) return 0.0;
+            const double th = side ? A.t.th[b.nt - 1] : A.t.th[0];
+            return ((th - PI_ / 2.0) * A.r) * A.sh;
+        }
+        return v;
+    }
+    // dir == 2
+    if (comp == 2) {
+        if (mode == 0) return ((2.0 * (A.t.th[j] - PI_ / 2.0)) * A.r) * A.sh - v;
+        if (mode == 1) return -v;
+        const double ph = side ? A.t.ph[b.np - 1] : A.t.ph[0];
+        return 2.0 * ((ph * A.r) * A.sh) - v;
+    }
+    if (mode == 0) return 0.0;
+    return v;
+}
+
+SSPT_DH inline bool bc_is_const(const Args &A, int comp, int dir)
+{
+    if (A.bc_mode != 0) return false;
+    if (dir == 1) return comp != 1;
+    if (dir == 2) return comp != 2;
+    return false;
+}
+
+SSPT_DH inline double ghost_value(const Args &A, const double *V, int comp, int i, int j, int k)
+{
+    const BoxG &b = A.b;
+    int ii[3] = {i, j, k};
+    int dirs[3], sides[3], nd = 0;
+    // collect directions from last (phi) to first (r); stop at a constant BC
+    bool konst = false;
+    for (int d = 2; d >= 0; --d) {
+        int inw;
+        if (ghost_dir(b, comp, d, ii[d], &inw)) {
+            dirs[nd] = d;
+            sides[nd] = ii[d] == 0 ? 0 : 1;
+            ++nd;
+            ii[d] = inw;
+            if (bc_is_const(A, comp, d)) { konst = true; break; }
+        }
+    }
+    double v = konst ? 0.0 : V[b.off(ii[0], ii[1], ii[2])];
+    for (int q = nd - 1; q >= 0; --q) {
+        // j of the position at which this BC is applied (ii[1] restored up to this level)
+        v = bc_apply(A, comp, dirs[q], sides[q], j, v);
+    }
+    return v;
+}
+
+// tII of interior cell (i,j,k) from the new stresses (compute_TII!)
+SSPT_DH inline double tii_inner(const Args &a, int i, int j, int k)
+{
+    const BoxG &b = a.b;
+    const long long o = b.off(i, j, k);
+    const double A = a.oRR[o], B = a.oTT[o], C = a.oPP[o];
+    const double *RT = a.oRT, *RP = a.oRP, *TP = a.oTP;
+    const double x = (((RT[o] + RT[o + b.sj]) + RT[o + b.si]) + RT[o + b.si + b.sj]) * 0.25;
+    const double y = (((RP[o] + RP[o + 1]) + RP[o + b.si]) + RP[o + b.si + 1]) * 0.25;
+    const double z = (((TP[o] + TP[o + 1]) + TP[o + b.sj]) + TP[o + b.sj + 1]) * 0.25;
+    return std::sqrt((((1.0 / 2.0) * ((A * A + B * B) + C * C) + x * x) + y * y) + z * z);
+}
+
+SSPT_DH inline void ghost_v_point(const Args &a, int comp, int i, int j, int k, unsigned long long *mdv,
+                                  unsigned long long *mv)
+{
+    const BoxG &b = a.b;
+    const double *Vn = comp == 0 ? a.oVR : (comp == 1 ? a.oVT : a.oVP);
+    double *Vw = comp == 0 ? a.oVR : (comp == 1 ? a.oVT : a.oVP);
+    const double *Vo = comp == 0 ? a.VR : (comp == 1 ? a.VT : a.VP);
+    double *Ve = comp == 0 ? a.VRe : (comp == 1 ? a.VTe : a.VPe);
+    const double v = ghost_value(a, Vn, comp, i, j, k);
+    const long long o = b.off(i, j, k);
+    Vw[o] = v;
+    if (a.flags & F_CHECK) {
+        const double d = v - Vo[o];
+        Ve[o] = d;
+        const unsigned long long ud = absbits(d), uv = absbits(v);
+        *mdv = ud > *mdv ? ud : *mdv;
+        *mv = uv > *mv ? uv : *mv;
+    } else if (a.flags & F_LAST) {
+        Ve[o] = Vo[o];
+    }
+}
+
+SSPT_DH inline void boundary_eta_point(const Args &a, int i, int j, int k)
+{
+    const BoxG &b = a.b;
+    const int ci = clampi(i, 1, b.nr - 2), cj = clampi(j, 1, b.nt - 2), ck = clampi(k, 1, b.np - 2);
+    const double tii = tii_inner(a, ci, cj, ck);
+    const long long o = b.off(i, j, k);
+    double pl;
+    const double oldpl = (a.flags & F_RELAX) ? a.ePL[o] : 0.0;
+    const double e = power_law_pt(tii, a.ini[o], oldpl, &pl, a);
+    a.oEta[o] = e;
+    if (a.flags & F_RELAX) a.oPL[o] = pl;
+    if (a.flags & F_LAST) a.TII[o] = tii;
+}
+
+} // namespace fused
+} // namespace sspt

@@ -1,0 +1,91 @@
+// sspt_internal.h -- device layout, metric tables and context of libsspt.so.
+//
+// Device layout ("box"): every field lives in a box of (nr+1) x (nth+1) x pitch
+// doubles, phi fastest, so all fields share the same strides:
+//     off(i,j,k) = i*si + j*sj + k,   sj = pitch, si = (nth+1)*pitch.
+// Index conventions (0-based):
+//   centre fields, VR, Vth, Vph: box (i,j,k) = reference (ix-1, iy-1, iz-1);
+//   shear stresses tau_Rth, tau_Rph, tau_thph: box (i,j,k) = reference (ix,iy,iz),
+//   i.e. an edge is indexed by the face index in its staggered directions and
+//   by the centre index otherwise (tau_Rth(i,j,k) sits on r-face i, th-face j,
+//   phi-centre k).  With this shift every stencil below is written in the same
+//   natural (i,j,k) frame.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/sspt.h"
+
+namespace sspt {
+
+struct Box {
+    int nr, nt, np;   // cells in r, th, phi
+    int pitch;        // phi row length (>= np+1)
+    long long sj, si; // strides of j and i
+    __host__ __device__ inline long long off(int i, int j, int k) const
+    {
+        return (long long)i * si + (long long)j * sj + k;
+    }
+};
+
+// 1D/2D FP64 metric tables, built on the host with the reference's exact
+// expression order (build_tables in sspt_tables.h), stored in one device
+// allocation.  r-tables have nr+1 entries, th-tables nth+1, 2D tables are
+// (nr+1) x (nth+1) with th fastest.
+struct Tables {
+    // r direction
+    const double *Rc;     // R_i                                        (centre)
+    const double *invRc;  // 1.0/R_i
+    const double *Rf;     // (R[i-1]+R[i])*0.5                          (@avxi_R, r face)
+    const double *invRf;
+    const double *Re;     // (((R[i-1]+R[i-1])+R[i])+R[i])*0.25         (@avxyi_R/@avxzi_R, r-edge)
+    const double *invRe;
+    const double *Rc4;    // (((R[i]+R[i])+R[i])+R[i])*0.25             (@avyzi_R)
+    const double *invRc4;
+    const double *Rc2;    // (R[i]+R[i])*0.5                            (@avyi_R/@avzi_R)
+    const double *invRc2;
+    // th direction
+    const double *cotc;   // cot(th_j) = 1/tan(th_j)
+    const double *cotf4;  // cot((((th[j-1]+th[j-1])+th[j])+th[j])*0.25)  (@avyzi_th)
+    const double *cotc2;  // cot((th[j]+th[j])*0.5)                        (@avxi_th/@avzi_th)
+    const double *cotf2;  // cot((th[j-1]+th[j])*0.5)                      (@avyi_th)
+    // 2D (1/R)/sin(th) combinations, indexed [i*(nth+1)+j]
+    const double *rs_c;   // (1/R_i)/sin(th_j)                compute_P centre
+    const double *rs_e;   // (1/Re_i)/sin(th4c_j)             tau_Rph edge
+    const double *rs_c4;  // (1/Rc4_i)/sin(th4f_j)            tau_thph edge
+    const double *rs_f;   // (1/Rf_i)/sin(th2c_j)             VR face
+    const double *rs_y;   // (1/Rc2_i)/sin(th2f_j)            Vth face
+    const double *rs_z;   // (1/Rc2_i)/sin(th2c_j)            Vph face
+    // coordinates for BCs / setup
+    const double *th;     // th_j
+    const double *ph;     // ph_k (nph+1 entries)
+};
+
+struct FieldSet {
+    double *f[SS_NFIELDS];
+};
+
+} // namespace sspt
+
+struct ss_ctx {
+    int device = 0;
+    sspt::Box box{};
+    size_t box_elems = 0;
+    hipStream_t stream = nullptr;
+    sspt::FieldSet cur{};   // current state
+    sspt::FieldSet alt{};   // ping-pong partners used by the fused loop
+    bool alt_ready = false;
+    double *tab_mem = nullptr;
+    sspt::Tables tab{};
+    bool grid_set = false;
+    double *staging = nullptr;      // upload/download staging (largest field)
+    size_t staging_elems = 0;
+    unsigned long long *red = nullptr; // reduction slots (device)
+    unsigned long long *red_host = nullptr; // pinned mirror
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> ev_pool;       // per-launch timing events
+    double last_kernel_ms = 0.0;
+    std::string err;
+};

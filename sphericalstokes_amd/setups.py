@@ -1,0 +1,180 @@
+"""Model setups: the diapir and plateau configurations as plain numpy state.
+
+A setup returns a :class:`Setup` holding the 1D coordinates, every field in
+the reference's Julia layout (Fortran-ordered float64 arrays, r fastest) and
+the scalar parameters of the PT iteration, so that the same input can be
+given to the HIP library and to the CPU oracle.
+
+* :func:`diapir` follows ``SPH_3D_Inclusion`` (SphericalStokes_diapir.jl:347-404)
+  with every constant a keyword; its defaults are the script's values.
+* :func:`diapir_sector` is the diapir in a wide sector r in [1,2],
+  th in [pi/3, 2pi/3], phi in [-pi/6, pi/6] (rising sphere, radius 0.1).
+* :func:`plateau_sector` is a thin-shell plateau load: r in [0.9, 1.0],
+  a power-law crust of thickness 0.01 carrying a tanh-edged 0.005 bump over
+  the central 60 % of a 30 deg x 40 deg sector, a linear mantle below.
+
+Coordinates come from ``numpy.linspace``; Julia's ``range`` may round the
+last bit of some points differently.  The library takes the caller's
+coordinates, so a Julia caller passes its own.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .fields import BC_FREE_SLIP, BC_PURE_SHEAR, BC_SHEAR, FIELDS, shape
+
+
+@dataclass
+class Params:
+    """Scalars of one PT iteration (names as in the reference scripts)."""
+    _dr: float
+    _dth: float
+    _dph: float
+    dmp: float
+    vpdt: float
+    lr: float
+    tau_C: float = 1.0
+    n_exp: float = 1.0
+    g: float = 1.0
+    r: float = 1.0
+    sh: float = 0.0
+    relax: float = 1.0
+    eta_min: float = 0.0
+    eta_max: float = math.inf
+    bc_mode: int = BC_SHEAR
+    err_drop_nan_vph: int = 0
+
+
+@dataclass
+class Setup:
+    nr: int
+    nth: int
+    nph: int
+    R: np.ndarray
+    th: np.ndarray
+    ph: np.ndarray
+    params: Params
+    fields: dict = field(default_factory=dict)
+    name: str = ""
+
+    def copy_fields(self) -> dict:
+        return {k: np.array(v, order="F", copy=True) for k, v in self.fields.items()}
+
+
+def _zeros(nr, nth, nph):
+    return {n: np.zeros(shape(n, nr, nth, nph), dtype=np.float64, order="F") for n in FIELDS}
+
+
+def _inclusion(A, R, th, ph, r, radius, val):
+    """initialize_inclusion! (SphericalStokes_diapir.jl:127-137): squared
+    distance compared with `radius`."""
+    a = (R - r)[:, None, None]
+    b = ((th - math.pi / 2.0) * r)[None, :, None]
+    c = (ph * r)[None, None, :]
+    d = (a * a + b * b) + c * c
+    A[d < radius] = val
+
+
+def _velocity_shear(Vph, th, r, fact):
+    """initialize_velocity!(Vph, th, sh, r) (SphericalStokes_diapir.jl:139-146)."""
+    Vph[...] = (((th - math.pi / 2.0) * r) * fact)[None, :, None]
+
+
+def diapir(nr, nth, nph, *, eta0=1.0, vr=1e2, rho0=0.0, drho=-10.0, g=1.0, n_exp=5.0, tau_C=1.0,
+           r=1000.0, lr=6.0, lth=None, lph=None, radius=1.0, sh=1.0, dmp=4.5, bc_mode=BC_SHEAR,
+           relax=1.0, eta_min=0.0, eta_max=math.inf, init_shear=None) -> Setup:
+    """SPH_3D_Inclusion setup, SphericalStokes_diapir.jl:350-404."""
+    lth = lr / r if lth is None else lth
+    lph = lr / r if lph is None else lph
+    CFL = 1.0 / (2.0 + 4.5 * math.log10(vr))
+    dr = lr / (nr - 1)
+    R = np.linspace(r - lr / 2.0, r + lr / 2.0, nr)
+    th = np.linspace(math.pi / 2.0 - lth / 2.0, math.pi / 2.0 + lth / 2.0, nth)
+    ph = np.linspace(-lph / 2.0, lph / 2.0, nph)
+    f = _zeros(nr, nth, nph)
+    f["rho"][...] = rho0
+    f["eta"][...] = eta0
+    _inclusion(f["rho"], R, th, ph, r, radius, rho0 + drho)
+    _inclusion(f["eta"], R, th, ph, r, radius, eta0 / vr)
+    if init_shear if init_shear is not None else bc_mode == BC_SHEAR:
+        _velocity_shear(f["Vph"], th, r, sh)
+    f["eta_ini"][...] = f["eta"]
+    f["eta_PL"][...] = f["eta"]
+    p = Params(_dr=(nr - 1) / lr, _dth=(nth - 1) / lth, _dph=(nph - 1) / lph, dmp=dmp, vpdt=dr * CFL,
+               lr=lr, tau_C=tau_C, n_exp=n_exp, g=g, r=r, sh=sh, relax=relax, eta_min=eta_min,
+               eta_max=eta_max, bc_mode=bc_mode)
+    return Setup(nr, nth, nph, R, th, ph, p, f, name="diapir")
+
+
+def diapir_sector(n, *, n_exp=1.0, vr=1e3, bc_mode=BC_FREE_SLIP, sh=0.0, relax=1.0, eta_min=0.0,
+                  eta_max=math.inf, nth=None, nph=None) -> Setup:
+    """Rising sphere in the sector r in [1,2], th in [pi/3,2pi/3], phi in [-pi/6,pi/6]:
+    sphere of radius 0.1 at (1.5, pi/2, 0), drho = -1, g = 1, eta_in/eta_m = 1/vr."""
+    s = diapir(n, nth or n, nph or n, eta0=1.0, vr=vr, rho0=0.0, drho=-1.0, g=1.0, n_exp=n_exp, tau_C=1.0,
+               r=1.5, lr=1.0, lth=math.pi / 3.0, lph=math.pi / 3.0, radius=0.1 ** 2, sh=sh, dmp=4.5,
+               bc_mode=bc_mode, relax=relax, eta_min=eta_min, eta_max=eta_max)
+    s.name = "diapir_sector"
+    return s
+
+
+def plateau_sector(n, *, nth=None, nph=None, n_crust=3.0, eta_mantle=1e-2, rho_crust=1.0, rho_mantle=1.2,
+                   eta_air=1e-3, dmp=4.5) -> Setup:
+    """Thin-shell plateau: r in [0.9, 1.0] (planet radius 1), th span 30 deg,
+    phi span 40 deg, a power-law crust 0.01 thick under a surface raised by a
+    tanh-edged 0.005 bump over the central 60 % of the sector; linear mantle
+    (viscosity ratio eta_mantle) below, weak low-density air above the surface.
+    Free slip on all faces."""
+    nth = nth or n
+    nph = nph or n
+    r_top, r_bot = 1.0, 0.9
+    lr = r_top - r_bot
+    lth, lph = math.radians(30.0), math.radians(40.0)
+    r_c = 0.5 * (r_top + r_bot)
+    R = np.linspace(r_bot, r_top, n)
+    th = np.linspace(math.pi / 2.0 - lth / 2.0, math.pi / 2.0 + lth / 2.0, nth)
+    ph = np.linspace(-lph / 2.0, lph / 2.0, nph)
+    w_th, w_ph = 0.03 * lth, 0.03 * lph
+    bth = 0.25 * (1 + np.tanh((th - (math.pi / 2 - 0.3 * lth)) / w_th)) * (1 - np.tanh((th - (math.pi / 2 + 0.3 * lth)) / w_th))
+    bph = 0.25 * (1 + np.tanh((ph + 0.3 * lph) / w_ph)) * (1 - np.tanh((ph - 0.3 * lph) / w_ph))
+    surf = 0.99 + 0.005 * bth[:, None] * bph[None, :]         # surface radius over (th, phi)
+    moho = surf - 0.01                                        # base of the crust
+    Rg = R[:, None, None]
+    crust = (Rg <= surf[None]) & (Rg > moho[None])
+    air = Rg > surf[None]
+    f = _zeros(n, nth, nph)
+    f["rho"][...] = rho_mantle
+    f["eta"][...] = eta_mantle
+    f["rho"][crust] = rho_crust
+    f["eta"][crust] = 1.0
+    f["rho"][air] = 0.0
+    f["eta"][air] = eta_air
+    f["eta_ini"][...] = f["eta"]
+    f["eta_PL"][...] = f["eta"]
+    vr = float(f["eta"].max() / f["eta"].min())
+    CFL = 1.0 / (2.0 + 4.5 * math.log10(vr))
+    dr = lr / (n - 1)
+    # The reference power law has one exponent for the whole domain.  Where
+    # tII << tau_C it tends to eta = 2*eta_ini (linear), which is the regime
+    # of the mantle and air here; the crust carries the stresses that reach
+    # tau_C and so the n_crust rheology.
+    p = Params(_dr=(n - 1) / lr, _dth=(nth - 1) / lth, _dph=(nph - 1) / lph, dmp=dmp, vpdt=dr * CFL, lr=lr,
+               tau_C=1.0, n_exp=n_crust, g=1.0, r=r_c, sh=0.0, bc_mode=BC_FREE_SLIP)
+    return Setup(n, nth, nph, R, th, ph, p, f, name="plateau_sector")
+
+
+def random_state(s: Setup, seed: int = 0, scale: float = 1e-2) -> Setup:
+    """Fills V, P and stresses with seeded random values (parity tests on a
+    non-trivial state without running a solve first)."""
+    rng = np.random.default_rng(seed)
+    for n in ("VR", "Vth", "Vph", "P", "tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph"):
+        a = s.fields[n]
+        a[...] = scale * rng.standard_normal(a.shape)
+    e = s.fields["eta"]
+    e[...] = np.exp(rng.standard_normal(e.shape))
+    s.fields["eta_ini"][...] = np.exp(rng.standard_normal(e.shape))
+    s.fields["eta_PL"][...] = s.fields["eta"]
+    s.fields["rho"][...] = rng.standard_normal(e.shape)
+    return s

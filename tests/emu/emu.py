@@ -1,0 +1,78 @@
+"""ctypes binding of the host emulation of the fused kernel (test support)."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+from sphericalstokes_amd._lib import ss_params
+from sphericalstokes_amd.fields import FIELDS, STATE
+
+EMU_LIB = Path(__file__).resolve().parent / "libsspt_emu.so"
+SHEAR = ("tau_Rth", "tau_Rph", "tau_thph")
+PINGPONG = set(STATE)
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not EMU_LIB.exists():
+            raise RuntimeError(f"{EMU_LIB} not built")
+        L = C.CDLL(str(EMU_LIB))
+        DP = C.POINTER(C.c_double)
+        L.emu_fused_iterations.restype = C.c_int
+        L.emu_fused_iterations.argtypes = [C.c_int] * 4 + [DP] * 3 + [C.POINTER(ss_params), C.POINTER(DP),
+                                           C.POINTER(DP), C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(C.c_ulonglong)]
+        _L = L
+    return _L
+
+
+def pitch_of(nph):
+    return ((nph + 1) + 63) // 64 * 64
+
+
+def to_box(setup, fields):
+    nr, nt, np_ = setup.nr, setup.nth, setup.nph
+    P = pitch_of(np_)
+    out = {}
+    for n in FIELDS:
+        b = np.zeros((nr + 1, nt + 1, P))
+        a = fields[n]
+        o = 1 if n in SHEAR else 0
+        b[o:o + a.shape[0], o:o + a.shape[1], o:o + a.shape[2]] = a
+        out[n] = b
+    return out
+
+
+def from_box(setup, box, shapes):
+    out = {}
+    for n in FIELDS:
+        o = 1 if n in SHEAR else 0
+        s = shapes[n]
+        out[n] = np.asfortranarray(box[n][o:o + s[0], o:o + s[1], o:o + s[2]])
+    return out
+
+
+def run(setup, fields, params, niter, check_last=False, rchunk=0):
+    """Runs the fused schedule on the host; returns (fields, red[6])."""
+    L = lib()
+    cur = to_box(setup, fields)
+    alt = {n: (np.zeros_like(cur[n]) if n in PINGPONG else cur[n]) for n in FIELDS}
+    DP = C.POINTER(C.c_double)
+    pc = (DP * len(FIELDS))(*[cur[n].ctypes.data_as(DP) for n in FIELDS])
+    pa = (DP * len(FIELDS))(*[alt[n].ctypes.data_as(DP) for n in FIELDS])
+    red = (C.c_ulonglong * 6)()
+    sp = ss_params.from_params(params)
+    R, th, ph = (np.ascontiguousarray(x, dtype=np.float64) for x in (setup.R, setup.th, setup.ph))
+    which = L.emu_fused_iterations(setup.nr, setup.nth, setup.nph, pitch_of(setup.nph), R.ctypes.data_as(DP),
+                                   th.ctypes.data_as(DP), ph.ctypes.data_as(DP), C.byref(sp), pc, pa,
+                                   int(niter), int(check_last), int(rchunk), red)
+    box = cur if which == 0 else alt
+    shapes = {n: fields[n].shape for n in FIELDS}
+    reds = np.array(list(red), dtype=np.uint64).view(np.float64)
+    return from_box(setup, box, shapes), reds

@@ -1,0 +1,167 @@
+// fused_emu.cpp -- host emulation of the fused PT iteration (test support).
+//
+// Compiles the device phase code of sphericalstokes_amd/csrc/sspt_fused_body.h
+// with the host compiler and runs it workgroup by workgroup: every thread of a
+// block executes a phase before any thread starts the next one, which is what
+// the barriers between phases guarantee on the GPU.  Tests use it to check the
+// fused schedule (tiling, halos, r-march, carries, boundary pass) against the
+// CPU oracle without a GPU.  It is never part of the product library.
+#define SSPT_DH
+#include "../../sphericalstokes_amd/csrc/sspt_fused_body.h"
+#include "../../sphericalstokes_amd/csrc/sspt_tables.h"
+#include "../../include/sspt.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <utility>
+#include <vector>
+
+using namespace sspt::fused;
+
+namespace {
+
+struct HostBox {
+    int nr, nt, np;
+    long long sj, si;
+};
+
+void run_main(const Args &a, int rchunk, unsigned long long red[6])
+{
+    const int nx = (a.b.np + OK_ - 1) / OK_;
+    const int ny = (a.b.nt + OJ - 1) / OJ;
+    const int nz = (a.b.nr + rchunk - 1) / rchunk;
+    std::unique_ptr<Lds> L(new Lds());
+    std::vector<Thread> th(TK * TJB);
+    for (int bz = 0; bz < nz; ++bz)
+        for (int by = 0; by < ny; ++by)
+            for (int bx = 0; bx < nx; ++bx) {
+                memset(L.get(), 0xff, sizeof(Lds)); // NaN garbage: unset LDS reads must not reach outputs
+                const int kb = bx * OK_, jb = by * OJ;
+                const int s0 = bz * rchunk, s1 = std::min(s0 + rchunk, a.b.nr);
+                const int start = std::max(s0 - 2, -1);
+                for (int ty = 0; ty < TJB; ++ty)
+                    for (int tx = 0; tx < TK; ++tx) {
+                        Thread &T = th[ty * TK + tx];
+                        thread_init(T, a, tx, ty, kb, jb);
+                        thread_prologue(T, a, start);
+                    }
+                for (int s = start; s < s1; ++s) {
+                    const bool wr = s >= s0;
+                    for (auto &T : th) phase1(T, *L, a, s);
+                    for (auto &T : th) phase2(T, *L, a, s, wr);
+                    for (auto &T : th) phase3(T, *L, a, s, wr);
+                    for (auto &T : th) phase4(T, *L, a, s, wr);
+                }
+                for (auto &T : th)
+                    for (int c = 0; c < 3; ++c) {
+                        red[c] = std::max(red[c], T.mdv[c]);
+                        red[3 + c] = std::max(red[3 + c], T.mv[c]);
+                    }
+            }
+}
+
+void run_boundary(const Args &a, unsigned long long red[6])
+{
+    const int nr = a.b.nr, nt = a.b.nt, np = a.b.np;
+    const int m = std::max(std::max(nr, nt), np) + 1;
+    for (int fam = 0; fam < 3; ++fam)
+        for (int v = 0; v < m; ++v)
+            for (int u = 0; u < m; ++u) {
+                unsigned long long mdv[3] = {0, 0, 0}, mv[3] = {0, 0, 0};
+                for (int c = 0; c < 3; ++c) {
+                    const int n0 = c == 0 ? nr + 1 : nr, n1 = c == 1 ? nt + 1 : nt, n2 = c == 2 ? np + 1 : np;
+                    if (fam == 0 && u < n2 && v < n1) {
+                        ghost_v_point(a, c, 0, v, u, &mdv[c], &mv[c]);
+                        ghost_v_point(a, c, n0 - 1, v, u, &mdv[c], &mv[c]);
+                    } else if (fam == 1 && u < n2 && v < n0) {
+                        ghost_v_point(a, c, v, 0, u, &mdv[c], &mv[c]);
+                        ghost_v_point(a, c, v, n1 - 1, u, &mdv[c], &mv[c]);
+                    } else if (fam == 2 && u < n1 && v < n0) {
+                        ghost_v_point(a, c, v, u, 0, &mdv[c], &mv[c]);
+                        ghost_v_point(a, c, v, u, n2 - 1, &mdv[c], &mv[c]);
+                    }
+                }
+                if (fam == 0 && u < np && v < nt) {
+                    boundary_eta_point(a, 0, v, u);
+                    boundary_eta_point(a, nr - 1, v, u);
+                } else if (fam == 1 && u < np && v >= 1 && v < nr - 1) {
+                    boundary_eta_point(a, v, 0, u);
+                    boundary_eta_point(a, v, nt - 1, u);
+                } else if (fam == 2 && u >= 1 && u < nt - 1 && v >= 1 && v < nr - 1) {
+                    boundary_eta_point(a, v, u, 0);
+                    boundary_eta_point(a, v, u, np - 1);
+                }
+                for (int c = 0; c < 3; ++c) {
+                    red[c] = std::max(red[c], mdv[c]);
+                    red[3 + c] = std::max(red[3 + c], mv[c]);
+                }
+            }
+}
+
+} // namespace
+
+extern "C" {
+
+// Runs `niter` fused iterations on box-layout buffers (shape (nr+1) x (nth+1)
+// x pitch, phi fastest).  cur/alt: SS_NFIELDS pointers each; non-state fields
+// must alias between the two sets.  The last iteration writes the auxiliary
+// fields; with check_last it is also a convergence-check iteration whose
+// max-reductions land in red[6].  Returns 0 if the final state is in `cur`,
+// 1 if it is in `alt`.
+int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, const double *th1, const double *ph1,
+                         const ss_params *p, double **cur, double **alt, int niter, int check_last, int rchunk,
+                         unsigned long long red[6])
+{
+    HostBox hb{nr, nt, np, pitch, (long long)(nt + 1) * pitch};
+    std::vector<double> tab;
+    sspt::build_tables(hb, R1, th1, ph1, tab);
+    const size_t n1 = nr + 1, n2 = nt + 1;
+    const double *q = tab.data();
+    auto take = [&](size_t n) { const double *r = q; q += n; return r; };
+    TabsG t;
+    t.Rc = take(n1); t.invRc = take(n1); t.Rf = take(n1); t.invRf = take(n1);
+    t.Re = take(n1); t.invRe = take(n1); t.Rc4 = take(n1); t.invRc4 = take(n1);
+    t.Rc2 = take(n1); t.invRc2 = take(n1);
+    t.cotc = take(n2); t.cotf4 = take(n2); t.cotc2 = take(n2); t.cotf2 = take(n2);
+    t.rs_c = take(n1 * n2); t.rs_e = take(n1 * n2); t.rs_c4 = take(n1 * n2);
+    t.rs_f = take(n1 * n2); t.rs_y = take(n1 * n2); t.rs_z = take(n1 * n2);
+    t.th = take(n2); t.ph = take(np + 1);
+    static constexpr double PI = 3.14159265358979323846;
+    for (int c = 0; c < 6; ++c) red[c] = 0ull;
+    double **in = cur, **out = alt;
+    int which = 0;
+    for (int it = 0; it < niter; ++it) {
+        const bool last = it == niter - 1, check = last && check_last;
+        Args a;
+        a.b = BoxG{nr, nt, np, hb.sj, hb.si};
+        a.t = t;
+        a.VR = in[SS_VR]; a.VT = in[SS_VTH]; a.VP = in[SS_VPH]; a.eta = in[SS_ETA];
+        a.P = in[SS_P]; a.tRR = in[SS_TAU_RR]; a.tTT = in[SS_TAU_THTH]; a.tPP = in[SS_TAU_PHPH];
+        a.tRT = in[SS_TAU_RTH]; a.tRP = in[SS_TAU_RPH]; a.tTP = in[SS_TAU_THPH];
+        a.rho = in[SS_RHO]; a.ini = in[SS_ETA_INI]; a.ePL = in[SS_ETA_PL];
+        a.oVR = out[SS_VR]; a.oVT = out[SS_VTH]; a.oVP = out[SS_VPH]; a.oEta = out[SS_ETA];
+        a.oP = out[SS_P]; a.oRR = out[SS_TAU_RR]; a.oTT = out[SS_TAU_THTH]; a.oPP = out[SS_TAU_PHPH];
+        a.oRT = out[SS_TAU_RTH]; a.oRP = out[SS_TAU_RPH]; a.oTP = out[SS_TAU_THPH]; a.oPL = out[SS_ETA_PL];
+        a.SM = out[SS_ETASM]; a.DT = out[SS_DT_R]; a.TII = out[SS_TII];
+        a.VRe = out[SS_VR_ERR]; a.VTe = out[SS_VTH_ERR]; a.VPe = out[SS_VPH_ERR];
+        a.red = nullptr;
+        a._dr = p->_dr; a._dth = p->_dth; a._dph = p->_dph;
+        a.kb = (((p->dmp * 2.0) * PI) * p->vpdt) / p->lr;
+        a.gs = ((4.0 * PI) * p->vpdt) / p->lr;
+        a.vpdt2 = p->vpdt * p->vpdt;
+        a.g = p->g; a.tC = p->tau_C; a.ex = 1.0 - p->n_exp; a.relax = p->relax;
+        a.emin = p->eta_min; a.emax = p->eta_max;
+        a.flags = (last ? F_LAST : 0) | (check ? F_CHECK : 0) | (p->relax != 1.0 ? F_RELAX : 0) |
+                  (p->eta_min > 0.0 ? F_CLAMP_LO : 0) | (p->eta_max < INFINITY ? F_CLAMP_HI : 0);
+        a.bc_mode = p->bc_mode; a.r = p->r; a.sh = p->sh;
+        a.rchunk = rchunk > 0 ? rchunk : nr;
+        run_main(a, a.rchunk, red);
+        run_boundary(a, red);
+        std::swap(in, out);
+        which ^= 1;
+    }
+    return which;
+}
+
+} // extern "C"

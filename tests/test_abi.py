@@ -1,0 +1,43 @@
+"""The C ABI: libsspt.so loads and exports every symbol include/sspt.h
+declares; argument checks that need no GPU return SS_ERR_ARG."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from sphericalstokes_amd import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    txt = (ROOT / "include" / "sspt.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(ss_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_symbols_exported():
+    L = C.CDLL(str(_lib.LIB_PATH))
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert set(declared_symbols()) == set(_lib.SIGNATURES)
+
+
+def test_create_rejects_bad_sizes_without_gpu():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.ss_create(2, 10, 10, 0, C.byref(h)) == _lib.SS_ERR_ARG
+    assert L.ss_create(10, 10, 10, 0, None) == _lib.SS_ERR_ARG
+    assert L.ss_destroy(None) == _lib.SS_ERR_ARG
+    assert L.ss_last_error(None) == b"null context"
+
+
+def test_params_struct_layout():
+    # ss_params: 14 doubles + 2 ints
+    assert C.sizeof(_lib.ss_params) == 14 * 8 + 2 * 4
+    assert C.sizeof(_lib.ss_result) == 4 + 4 + 8 + 24 + 8

@@ -1,0 +1,208 @@
+"""GPU parity: libsspt.so (HIP, gfx950) against the CPU oracle.
+
+Per-kernel entry points must match the oracle bit for bit (-ffp-contract=off,
+tables built with the reference association, correctly rounded IEEE division
+and sqrt); the only exception is power_law's pow(), whose device libm may
+differ by an ulp, so kernels that run it are compared within 4 ulp / 1e-14.
+The fused loop must match the oracle's iteration sequence bit for bit when
+pow is not involved (n_exp = 1 makes the exponent 0) and to rel-L2 <= 1e-10
+otherwise, the FP64 tolerance of the drop-in.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import compare_fields, max_ulp, rel_l2
+from oracle.binding import Oracle
+from sphericalstokes_amd import setups
+from sphericalstokes_amd.fields import BC_FREE_SLIP, BC_PURE_SHEAR, BC_SHEAR, FIELDS
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10      # north_star FP64 drop-in tolerance (rel-L2 per field)
+POW_ULP = 4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def Solver():
+    from sphericalstokes_amd.solver import Solver as S
+    return S
+
+
+def _gpu_state(S, setup, fields=None):
+    s = S(setup.nr, setup.nth, setup.nph)
+    s.set_grid(setup.R, setup.th, setup.ph)
+    f = fields if fields is not None else setup.fields
+    for n in FIELDS:
+        s.upload(n, f[n])
+    return s
+
+
+def _case(shape=(13, 70, 130), seed=1, kind="sector", **kw):
+    nr, nt, np_ = shape
+    if kind == "sector":
+        st = setups.diapir_sector(nr, nth=nt, nph=np_, **kw)
+    else:
+        st = setups.diapir(nr, nt, np_, vr=1e2, **kw)
+    return setups.random_state(st, seed=seed)
+
+
+def test_upload_download_roundtrip(Solver):
+    st = _case((9, 11, 70))
+    say("roundtrip", (9, 11, 70))
+    with _gpu_state(Solver, st) as s:
+        for n in FIELDS:
+            a = st.fields[n]
+            assert np.array_equal(s.download(n), a), n
+            c = s.download(n, order="C")
+            assert np.array_equal(c, a), n
+            s.upload(n, np.ascontiguousarray(a[::-1].copy()))   # C layout upload
+            assert np.array_equal(s.download(n), a[::-1]), n
+
+
+KERNELS = ["maxloc", "copy_bc_etaSM", "compute_P", "compute_TII", "copy_bc_tII", "power_law", "compute_V",
+           "bc_R", "bc_th", "bc_ph", "equal3"]
+
+
+@pytest.mark.parametrize("bc_mode", [BC_SHEAR, BC_FREE_SLIP, BC_PURE_SHEAR])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_per_kernel_bitwise(Solver, kernel, bc_mode):
+    st = _case((13, 70, 130), n_exp=3.0, bc_mode=bc_mode, sh=0.8)
+    p = st.params
+    say("kernel", kernel, "bc", bc_mode)
+    o = Oracle(st)
+    with _gpu_state(Solver, st) as s:
+        if kernel == "maxloc":
+            o.maxloc(); s.maxloc()
+        elif kernel == "copy_bc_etaSM":
+            o.copy_bc("etaSM"); s.copy_BC("etaSM")
+        elif kernel == "copy_bc_tII":
+            o.copy_bc("tII"); s.copy_BC("tII")
+        elif kernel == "compute_P":
+            o.compute_P(); s.compute_P(p._dr, p._dth, p._dph, p.dmp, p.vpdt, p.lr)
+        elif kernel == "compute_TII":
+            o.compute_TII(); s.compute_TII()
+        elif kernel == "power_law":
+            o.power_law(); s.power_law(p.tau_C, p.n_exp)
+        elif kernel == "compute_V":
+            o.compute_V(); s.compute_V(p._dr, p._dth, p._dph, p.g)
+        elif kernel == "bc_R":
+            o.bc_R(); s.BC_R()
+        elif kernel == "bc_th":
+            o.bc_th(); s.BC_th(p.bc_mode, p.r, p.sh)
+        elif kernel == "bc_ph":
+            o.bc_ph(); s.BC_ph(p.bc_mode, p.r, p.sh)
+        elif kernel == "equal3":
+            o.equal3(); s.equal3()
+        got = s.download_all()
+    if kernel == "power_law":
+        assert max_ulp(got["eta"], o.f["eta"]) <= POW_ULP
+        got["eta"] = o.f["eta"]
+    res, bad = compare_fields(got, o.f)
+    assert not bad, {k: res[k] for k in bad}
+
+
+def test_check_err_matches(Solver):
+    st = _case((9, 20, 66))
+    o = Oracle(st)
+    with _gpu_state(Solver, st) as s:
+        e, comp = s.check_err()
+        got = s.download_all()
+    e_ref, comp_ref = o.check_err()
+    assert e == e_ref and tuple(comp) == tuple(comp_ref)
+    res, bad = compare_fields(got, o.f)
+    assert not bad
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("case", ["linear_free_slip", "linear_shear"])
+def test_iterations_bitwise_linear(Solver, fused, case):
+    """n_exp = 1 -> pow(x, 0) = 1 exactly: the whole iteration is bitwise."""
+    bc = BC_FREE_SLIP if case == "linear_free_slip" else BC_SHEAR
+    st = _case((21, 40, 70), n_exp=1.0, bc_mode=bc, sh=1.0, seed=7)
+    say("iterations", case, "fused", fused)
+    o = Oracle(st)
+    o.step(6)
+    with _gpu_state(Solver, st) as s:
+        if fused:
+            s.pt_iterate(st.params, 6)
+        else:
+            for _ in range(6):
+                s.pt_step_unfused(st.params)
+        got = s.download_all()
+    res, bad = compare_fields(got, o.f)
+    assert not bad, {k: res[k] for k in bad}
+
+
+@pytest.mark.parametrize("kw", [dict(n_exp=3.0, bc_mode=BC_SHEAR, sh=1.0),
+                                dict(n_exp=3.0, bc_mode=BC_SHEAR, sh=1.0, relax=0.5, eta_min=1e-4, eta_max=1e4),
+                                dict(n_exp=3.0, bc_mode=BC_PURE_SHEAR, sh=1.0)])
+def test_fused_powerlaw_tolerance(Solver, kw):
+    st = _case((21, 40, 70), seed=8, **kw)
+    say("powerlaw fused", kw)
+    o = Oracle(st)
+    o.step(10)
+    with _gpu_state(Solver, st) as s:
+        s.pt_iterate(st.params, 10)
+        got = s.download_all()
+    for n in FIELDS:
+        assert rel_l2(got[n], o.f[n]) <= TOL_FP64, n
+
+
+def test_fused_equals_unfused_on_gpu(Solver):
+    """Fused and per-kernel GPU paths run the same device libm: bitwise."""
+    st = _case((33, 33, 33), n_exp=3.0, bc_mode=BC_SHEAR, sh=1.0, seed=9)
+    with _gpu_state(Solver, st) as a, _gpu_state(Solver, st) as b:
+        a.pt_iterate(st.params, 20)
+        for _ in range(20):
+            b.pt_step_unfused(st.params)
+        ga, gb = a.download_all(), b.download_all()
+    res, bad = compare_fields(ga, gb)
+    assert not bad, {k: res[k] for k in bad}
+
+
+def test_pt_solve_matches_oracle(Solver):
+    """The exit test, ittot quirk and err of the reference loop."""
+    st = setups.diapir_sector(15, n_exp=1.0, bc_mode=BC_FREE_SLIP)
+    st.params.err_drop_nan_vph = 0
+    o = Oracle(st)
+    ref = o.solve(20000, 100, 1e-6)
+    say("oracle solve", ref)
+    for fused in (True, False):
+        with _gpu_state(Solver, st) as s:
+            r = s.pt_solve(st.params, 20000, 100, 1e-6, fused=fused)
+            got = s.download_all()
+        say("gpu solve fused" if fused else "gpu solve unfused", r)
+        assert r["ittot"] == ref["ittot"] and r["iter_end"] == ref["iter_end"]
+        assert r["err"] == ref["err"]
+        res, bad = compare_fields(got, o.f)
+        assert not bad, {k: res[k] for k in bad}
+
+
+def test_nonfinite_status(Solver):
+    st = setups.diapir_sector(11, n_exp=1.0)
+    st.fields["VR"][5, 5, 5] = math.nan
+    with _gpu_state(Solver, st) as s:
+        r = s.pt_solve(st.params, 200, 100, 1e-8, raise_on=())
+    assert r["status"] == "SS_ERR_NONFINITE"
+
+
+def test_fullsize_255_against_oracle(Solver):
+    """255^3 diapir_sector with n = 3, cutoffs and relaxation: 4 fused
+    iterations against the oracle over the full arrays."""
+    st = setups.diapir_sector(255, n_exp=3.0, bc_mode=BC_SHEAR, sh=1.0, relax=0.5, eta_min=1e-4, eta_max=1e4)
+    say("255^3 oracle 4 steps")
+    o = Oracle(st)
+    o.step(4)
+    with _gpu_state(Solver, st) as s:
+        s.pt_iterate(st.params, 4)
+        for n in FIELDS:
+            assert rel_l2(s.download(n), o.f[n]) <= TOL_FP64, n
