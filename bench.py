@@ -65,6 +65,9 @@ def main():
     ap.add_argument("--n", type=int, default=511)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain field copy (k_copy_field: n^3 doubles read + written) so that "
+                         "rocprofv3 --pmc byte counters can be scaled")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the PT iteration runs one rank's block on one MI355X; "
@@ -81,6 +84,8 @@ def main():
     s.set_grid(st.R, st.th, st.ph)
     for name in ("rho", "eta", "eta_ini", "eta_PL", "VR", "Vth", "Vph"):
         s.upload(name, st.fields[name])      # everything else starts at zero, as allocated
+    if args.calibrate_copy:
+        s.copy_field("DT_R", "eta")          # DT_R is rewritten by the last iteration
     s.pt_iterate(p, args.warmup)
     t0 = time.perf_counter()
     main_ms, iter_ms = s.pt_iterate_timed(p, args.steps)

@@ -169,6 +169,10 @@ int ss_create(int nr, int nth, int nph, int device, ss_ctx **out)
     }
     if (hipMalloc(&c->tab_mem, tables_elems(b) * sizeof(double)) != hipSuccess) { ss_destroy(c); return SS_ERR_NOMEM; }
     bind_tables(c);
+    if (hipMalloc(&c->ftab_mem, fused_tables_elems(b) * sizeof(double)) != hipSuccess) { ss_destroy(c); return SS_ERR_NOMEM; }
+    c->ftr = c->ftab_mem;
+    c->ftt = c->ftr + (size_t)FT_NTR * (nr + 1);
+    c->ft2 = c->ftt + (size_t)FT_NTT * (nth + 1);
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
     rc = sync(c, "ss_create");
@@ -188,6 +192,7 @@ int ss_destroy(ss_ctx *c)
     if (c->red) (void)hipFree(c->red);
     if (c->red_host) (void)hipHostFree(c->red_host);
     if (c->tab_mem) (void)hipFree(c->tab_mem);
+    if (c->ftab_mem) (void)hipFree(c->ftab_mem);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -218,6 +223,9 @@ int ss_set_grid(ss_ctx *c, const double *R1, const double *th1, const double *ph
     std::vector<double> h;
     build_tables(c->box, R1, th1, ph1, h);
     HIP_TRY(c, hipMemcpyAsync(c->tab_mem, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> hf;
+    build_fused_tables(c->box, h, hf);
+    HIP_TRY(c, hipMemcpyAsync(c->ftab_mem, hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int rc = sync(c, "ss_set_grid");
     if (rc == SS_OK) c->grid_set = true;
     return rc;

@@ -48,6 +48,7 @@ __device__ __forceinline__ void tile_of(int nx, int ny, int nz, int &bx, int &by
     bz = t / (nx * ny);
 }
 
+template <int FL>
 __global__ void __launch_bounds__(TK * TJB) k_fused_main(Args a, int nx, int ny, int nz)
 {
     __shared__ Lds L;
@@ -59,20 +60,20 @@ __global__ void __launch_bounds__(TK * TJB) k_fused_main(Args a, int nx, int ny,
     const int start = max(s0 - 2, -1);
     Thread T;
     thread_init(T, a, threadIdx.x, threadIdx.y, kb, jb);
-    thread_prologue(T, a, start);
+    thread_prologue<FL>(T, a, start);
     for (int s = start; s < s1; ++s) {
         const bool wr = s >= s0;
-        phase1(T, L, a, s);
+        phase1<FL>(T, L, a, s);
         __syncthreads();
-        phase2(T, L, a, s, wr);
+        phase2<FL>(T, L, a, s, wr);
         __syncthreads();
-        phase3(T, L, a, s, wr);
+        phase3<FL>(T, L, a, s, wr);
         __syncthreads();
         // phase4 reads only STT/SPP/DT/TP/RT/RP; the next phase1 writes only
         // VR/VT/VP/ETA, so no barrier is needed between them.
-        phase4(T, L, a, s, wr);
+        phase4<FL>(T, L, a, s, wr);
     }
-    if (a.flags & F_CHECK) flush_max(a, T.mdv, T.mv, threadIdx.x);
+    if (FL & F_CHECK) flush_max(a, T.mdv, T.mv, threadIdx.x);
 }
 
 // Boundary pass.  blockIdx.z = family: 0 -> r faces over (j,k),
@@ -138,9 +139,7 @@ static Args make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out, const 
     a.b.np = c->box.np;
     a.b.sj = c->box.sj;
     a.b.si = c->box.si;
-    const Tables &t = c->tab;
-    a.t = TabsG{t.Rc, t.invRc, t.Rf, t.invRf, t.Re, t.invRe, t.Rc4, t.invRc4, t.Rc2, t.invRc2,
-                t.cotc, t.cotf4, t.cotc2, t.cotf2, t.rs_c, t.rs_e, t.rs_c4, t.rs_f, t.rs_y, t.rs_z, t.th, t.ph};
+    a.t = TabsG{c->ftr, c->ftt, c->ft2, c->tab.th, c->tab.ph};
     a.VR = in.f[SS_VR];
     a.VT = in.f[SS_VTH];
     a.VP = in.f[SS_VPH];
@@ -187,7 +186,7 @@ static Args make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out, const 
     a.emin = p.eta_min;
     a.emax = p.eta_max;
     a.flags = (last ? F_LAST : 0) | (check ? F_CHECK : 0) | (p.relax != 1.0 ? F_RELAX : 0) |
-              (p.eta_min > 0.0 ? F_CLAMP_LO : 0) | (p.eta_max < INFINITY ? F_CLAMP_HI : 0);
+              ((p.eta_min > 0.0 || p.eta_max < INFINITY) ? F_CLAMP : 0);
     a.bc_mode = p.bc_mode;
     a.r = p.r;
     a.sh = p.sh;
@@ -220,7 +219,14 @@ void launch_fused_iteration(ss_ctx *c, const FieldSet &in, const FieldSet &out, 
     const int T = nx * ny * nz;
     const int grid = ((T + 7) / 8) * 8;
     if (ev_main0) (void)hipEventRecord(ev_main0, c->stream);
-    hipLaunchKernelGGL(k_fused_main, dim3(grid), dim3(TK, TJB), 0, c->stream, a, nx, ny, nz);
+    const dim3 G(grid), B(TK, TJB);
+    switch (a.flags) {
+#define SSPT_FL(v) case v: hipLaunchKernelGGL(k_fused_main<v>, G, B, 0, c->stream, a, nx, ny, nz); break;
+    SSPT_FL(0) SSPT_FL(1) SSPT_FL(3) SSPT_FL(4) SSPT_FL(5) SSPT_FL(7)
+    SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15)
+#undef SSPT_FL
+    default: break; // CHECK without LAST is never requested
+    }
     if (ev_main1) (void)hipEventRecord(ev_main1, c->stream);
     const int m = std::max(std::max(a.b.nr, a.b.nt), a.b.np) + 1;
     hipLaunchKernelGGL(k_fused_boundary, dim3((m + 63) / 64, (m + 3) / 4, 3), dim3(64, 4), 0, c->stream, a);

@@ -26,6 +26,11 @@
 #ifndef SSPT_DH
 #define SSPT_DH __host__ __device__
 #endif
+// A value that is the same in every lane of a wavefront (blockDim.x == 64, so
+// threadIdx.y and everything derived from it is wave-uniform).
+#ifndef SSPT_UNIFORM
+#define SSPT_UNIFORM(x) __builtin_amdgcn_readfirstlane(x)
+#endif
 
 namespace sspt {
 namespace fused {
@@ -36,7 +41,7 @@ constexpr int OK_ = TK - 2;   // owned points per row
 constexpr int OJ = TJB - 2;   // owned rows
 constexpr double PI_ = 3.14159265358979323846;
 
-enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP_LO = 8, F_CLAMP_HI = 16 };
+enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP = 8 };
 
 struct BoxG {
     int nr, nt, np;
@@ -44,10 +49,17 @@ struct BoxG {
     SSPT_DH long long off(int i, int j, int k) const { return (long long)i * si + (long long)j * sj + k; }
 };
 
+// Interleaved FP64 metric tables (one record per index, see
+// build_fused_tables in sspt_tables.h): a record is read with a few scalar
+// loads because s and the wavefront's j are uniform.
+enum { TR_Rc, TR_invRc, TR_Rf, TR_invRf, TR_Re, TR_invRe, TR_Rc4, TR_invRc4, TR_Rc2, TR_invRc2, NTR = 16 };
+enum { TT_cotc, TT_cotf4, TT_cotc2, TT_cotf2, NTT = 4 };
+enum { T2_c, T2_e, T2_c4, T2_f, T2_y, T2_z, NT2 = 8 };
+
 struct TabsG {
-    const double *Rc, *invRc, *Rf, *invRf, *Re, *invRe, *Rc4, *invRc4, *Rc2, *invRc2;
-    const double *cotc, *cotf4, *cotc2, *cotf2;
-    const double *rs_c, *rs_e, *rs_c4, *rs_f, *rs_y, *rs_z;
+    const double *tr; // [i*NTR + TR_*], i in [0, nr]
+    const double *tt; // [j*NTT + TT_*], j in [0, nt]
+    const double *t2; // [(i*(nt+1)+j)*NT2 + T2_*]
     const double *th, *ph;
 };
 
@@ -95,16 +107,45 @@ SSPT_DH inline unsigned long long absbits(double x)
     return c.u;
 }
 
+// x / d for d with a correctly rounded reciprocal inv_d = RN(1/d): one FMA
+// correction of x*inv_d gives the correctly rounded quotient (Markstein's
+// theorem), i.e. exactly what IEEE division returns, for a product and two FMAs
+// instead of a full division sequence.
+SSPT_DH inline double div_rcp(double x, double d, double inv_d)
+{
+    const double q = x * inv_d;
+    const double r = std::fma(-q, d, x);
+    return std::fma(r, inv_d, q);
+}
+
+// x^ex of the power law.  ex = 0 (n_exp = 1) is 1 exactly, as pow returns;
+// small integer exponents (n_exp = 2..9 -> ex = -1..-8) use repeated
+// products and one reciprocal (within 2 ulp of pow); other exponents use pow.
+SSPT_DH inline double pow_ex(double x, double ex)
+{
+    if (ex == 0.0) return 1.0;
+    if (ex < 0.0 && ex >= -8.0 && ex == (double)(int)ex) {
+        const int m = -(int)ex;
+        double p = x;
+        for (int i = 1; i < m; ++i) p = p * x;
+        return 1.0 / p;
+    }
+    return std::pow(x, ex);
+}
+
 // power_law! (SphericalStokes_diapir.jl:259-265) with the optional log-space
 // relaxation (CartesianStokes_diapir.jl:210-221) and cutoffs.
-SSPT_DH inline double power_law_pt(double tii, double ini, double ePL_old, double *ePL_new, const Args &a)
+SSPT_DH inline double power_law_pt(double tii, double ini, double ePL_old, double *ePL_new, const Args &a,
+                                   int flags)
 {
-    double pl = ini * std::pow(tii / a.tC, a.ex);
-    if (a.flags & F_RELAX) pl = std::exp(std::log(pl) * a.relax + std::log(ePL_old) * (1.0 - a.relax));
+    double pl = ini * pow_ex(tii / a.tC, a.ex);
+    if (flags & F_RELAX) pl = std::exp(std::log(pl) * a.relax + std::log(ePL_old) * (1.0 - a.relax));
     *ePL_new = pl;
     double e = 2.0 / (1.0 / ini + 1.0 / pl);
-    if (a.flags & F_CLAMP_LO) e = jmax(e, a.emin);
-    if (a.flags & F_CLAMP_HI) e = jmin(e, a.emax);
+    if (flags & F_CLAMP) {
+        e = jmax(e, a.emin);   // eta_min = 0 / eta_max = +inf leave eta unchanged
+        e = jmin(e, a.emax);
+    }
     return e;
 }
 
@@ -115,6 +156,7 @@ struct Thread {
     // registers carried along r
     double vr0, vr1, vt0, vt1, vp0, vp1;     // old V at planes s, s+1 (own column)
     double e0, e1, e2;                       // old eta at s, s+1, s+2
+    double nvr, nvt, nvp, ne, ringJ, ringK;  // prefetched: V(s+2), eta(s+3), eta ring of plane s+2
     double sm0, sm1;                         // etaSM at s, s+1
     double cRR, cTT, cPP, cDT, crho;         // sigma/DT/rho of plane s-1 (VR face s)
     double rt0, rt0n, rp0, rp0n;             // new tau_Rth/tau_Rph of face s (own, +1 neighbour)
@@ -131,9 +173,9 @@ struct Thread {
 SSPT_DH inline void thread_init(Thread &T, const Args &a, int tx, int ty, int kb, int jb)
 {
     T.tx = tx;
-    T.ty = ty;
+    T.ty = SSPT_UNIFORM(ty);
     T.k = kb - 1 + tx;
-    T.j = jb - 1 + ty;
+    T.j = SSPT_UNIFORM(jb - 1 + ty);
     T.kc = clampi(T.k, 0, a.b.np);
     T.jc = clampi(T.j, 0, a.b.nt);
     T.own = tx >= 1 && tx <= TK - 2 && ty >= 1 && ty <= TJB - 2 && T.j < a.b.nt && T.k < a.b.np;
@@ -147,57 +189,103 @@ SSPT_DH inline void thread_init(Thread &T, const Args &a, int tx, int ty, int kb
     T.nrt = T.nrp = T.ntp = 0.0;
 }
 
-// Loads of the first plane(s) before the march (start = first step).
-SSPT_DH inline void thread_prologue(Thread &T, const Args &a, int start)
+// Row/column ring of the eta plane p for the star of the block's edge threads.
+SSPT_DH inline void load_ring(Thread &T, const Args &a, int p)
 {
     const BoxG &b = a.b;
-    const int i0 = clampi(start, 0, b.nr - 1), i1 = clampi(start + 1, 0, b.nr - 1);
-    T.e0 = a.eta[b.off(i0, clampi(T.jc, 0, b.nt - 1), clampi(T.kc, 0, b.np - 1))];
-    T.e1 = a.eta[b.off(i1, clampi(T.jc, 0, b.nt - 1), clampi(T.kc, 0, b.np - 1))];
-    const int iv = clampi(start, 0, b.nr);
-    T.vr1 = a.VR[b.off(iv, T.jc, T.kc)];
-    T.vt1 = a.VT[b.off(clampi(start, 0, b.nr - 1), T.jc, T.kc)];
-    T.vp1 = a.VP[b.off(clampi(start, 0, b.nr - 1), T.jc, T.kc)];
+    const int ie = clampi(p, 0, b.nr - 1);
+    const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
+    if (T.ty == 0) T.ringJ = a.eta[b.off(ie, clampi(T.j - 1, 0, b.nt - 1), kcc)];
+    if (T.ty == TJB - 1) T.ringJ = a.eta[b.off(ie, clampi(T.j + 1, 0, b.nt - 1), kcc)];
+    if (T.tx == 0) T.ringK = a.eta[b.off(ie, jcc, clampi(T.k - 1, 0, b.np - 1))];
+    if (T.tx == TK - 1) T.ringK = a.eta[b.off(ie, jcc, clampi(T.k + 1, 0, b.np - 1))];
 }
 
-// ---- phase 1: stage old V and eta of the coming planes, issue own loads
-SSPT_DH inline void phase1(Thread &T, Lds &L, const Args &a, int s)
+SSPT_DH inline long long own_centre(const Thread &T, const BoxG &b, int s)
 {
-    const BoxG &b = a.b;
-    const int nr = b.nr;
-    // shift the r window: values loaded for plane s+1 at step s-1 become plane s
-    T.vr0 = T.vr1;
-    T.vt0 = T.vt1;
-    T.vp0 = T.vp1;
-    T.vr1 = a.VR[b.off(clampi(s + 1, 0, nr), T.jc, T.kc)];
-    T.vt1 = a.VT[b.off(clampi(s + 1, 0, nr - 1), T.jc, T.kc)];
-    T.vp1 = a.VP[b.off(clampi(s + 1, 0, nr - 1), T.jc, T.kc)];
-    const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
-    T.e2 = a.eta[b.off(clampi(s + 2, 0, nr - 1), jcc, kcc)];
-    L.VR[T.ty][T.tx] = T.vr1;
-    L.VT[T.ty][T.tx] = T.vt0;
-    L.VP[T.ty][T.tx] = T.vp0;
-    // eta plane s+1 with a one-point ring for the 7-point star of the halo
-    const int ie = clampi(s + 1, 0, nr - 1);
-    L.ETA[T.ty + 1][T.tx + 1] = T.e1;
-    if (T.ty == 0) L.ETA[0][T.tx + 1] = a.eta[b.off(ie, clampi(T.j - 1, 0, b.nt - 1), kcc)];
-    if (T.ty == TJB - 1) L.ETA[TJB + 1][T.tx + 1] = a.eta[b.off(ie, clampi(T.j + 1, 0, b.nt - 1), kcc)];
-    if (T.tx == 0) L.ETA[T.ty + 1][0] = a.eta[b.off(ie, jcc, clampi(T.k - 1, 0, b.np - 1))];
-    if (T.tx == TK - 1) L.ETA[T.ty + 1][TK + 1] = a.eta[b.off(ie, jcc, clampi(T.k + 1, 0, b.np - 1))];
-    // own old values used by this step
-    const int ic = clampi(s, 0, nr - 1);
-    const long long oc = b.off(ic, jcc, kcc);
+    return b.off(clampi(s, 0, b.nr - 1), clampi(T.jc, 0, b.nt - 1), clampi(T.kc, 0, b.np - 1));
+}
+
+// Own old values of step s.  They are issued one step ahead (right after
+// their previous use), so their latency hides behind a whole step.
+SSPT_DH inline void load_centre_old(Thread &T, const Args &a, int s)
+{
+    const long long oc = own_centre(T, a.b, s);
     T.oP = a.P[oc];
     T.oRR = a.tRR[oc];
     T.oTT = a.tTT[oc];
     T.oPP = a.tPP[oc];
     T.orho = a.rho[oc];
-    T.oini = a.ini[oc];
-    T.oPL = (a.flags & F_RELAX) ? a.ePL[oc] : 0.0;
-    T.oTPold = a.tTP[b.off(ic, T.jc, T.kc)];
-    const int ie1 = clampi(s + 1, 0, nr);
-    T.oRTold = a.tRT[b.off(ie1, T.jc, T.kc)];
-    T.oRPold = a.tRP[b.off(ie1, T.jc, T.kc)];
+    T.oTPold = a.tTP[a.b.off(clampi(s, 0, a.b.nr - 1), T.jc, T.kc)];
+}
+SSPT_DH inline void load_face_old(Thread &T, const Args &a, int s)
+{
+    const long long oe = a.b.off(clampi(s + 1, 0, a.b.nr), T.jc, T.kc);
+    T.oRTold = a.tRT[oe];
+    T.oRPold = a.tRP[oe];
+}
+template <int FL> SSPT_DH inline void load_visc_old(Thread &T, const Args &a, int s)
+{
+    if (T.own) {
+        const long long oc = own_centre(T, a.b, s);
+        T.oini = a.ini[oc];
+        T.oPL = (FL & F_RELAX) ? a.ePL[oc] : 0.0;
+    }
+}
+
+// Loads before the march (start = first step): the V/eta window, the
+// prefetch registers and the own old values of step `start`.
+template <int FL> SSPT_DH inline void thread_prologue(Thread &T, const Args &a, int start)
+{
+    const BoxG &b = a.b;
+    const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
+    T.e0 = a.eta[b.off(clampi(start, 0, b.nr - 1), jcc, kcc)];
+    T.e1 = a.eta[b.off(clampi(start + 1, 0, b.nr - 1), jcc, kcc)];
+    T.ne = a.eta[b.off(clampi(start + 2, 0, b.nr - 1), jcc, kcc)];
+    T.vr1 = a.VR[b.off(clampi(start, 0, b.nr), T.jc, T.kc)];
+    T.vt1 = a.VT[b.off(clampi(start, 0, b.nr - 1), T.jc, T.kc)];
+    T.vp1 = a.VP[b.off(clampi(start, 0, b.nr - 1), T.jc, T.kc)];
+    T.nvr = a.VR[b.off(clampi(start + 1, 0, b.nr), T.jc, T.kc)];
+    T.nvt = a.VT[b.off(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc)];
+    T.nvp = a.VP[b.off(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc)];
+    T.ringJ = T.ringK = 0.0;
+    load_ring(T, a, start + 1);
+    load_centre_old(T, a, start);
+    load_face_old(T, a, start);
+    load_visc_old<FL>(T, a, start);
+}
+
+// ---- phase 1: stage old V and eta of the coming plane in LDS (values
+// prefetched during the previous step) and prefetch the plane after it
+template <int FL>
+SSPT_DH inline void phase1(Thread &T, Lds &L, const Args &a, int s)
+{
+    const BoxG &b = a.b;
+    const int nr = b.nr;
+    // shift the r window: plane s+1 values were prefetched at step s-1
+    T.vr0 = T.vr1;
+    T.vt0 = T.vt1;
+    T.vp0 = T.vp1;
+    T.vr1 = T.nvr;
+    T.vt1 = T.nvt;
+    T.vp1 = T.nvp;
+    T.e2 = T.ne;
+    L.VR[T.ty][T.tx] = T.vr1;
+    L.VT[T.ty][T.tx] = T.vt0;
+    L.VP[T.ty][T.tx] = T.vp0;
+    // eta plane s+1 with a one-point ring for the 7-point star of the halo
+    L.ETA[T.ty + 1][T.tx + 1] = T.e1;
+    if (T.ty == 0) L.ETA[0][T.tx + 1] = T.ringJ;
+    if (T.ty == TJB - 1) L.ETA[TJB + 1][T.tx + 1] = T.ringJ;
+    if (T.tx == 0) L.ETA[T.ty + 1][0] = T.ringK;
+    if (T.tx == TK - 1) L.ETA[T.ty + 1][TK + 1] = T.ringK;
+    // prefetch for step s+1
+    const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
+    T.nvr = a.VR[b.off(clampi(s + 2, 0, nr), T.jc, T.kc)];
+    T.nvt = a.VT[b.off(clampi(s + 2, 0, nr - 1), T.jc, T.kc)];
+    T.nvp = a.VP[b.off(clampi(s + 2, 0, nr - 1), T.jc, T.kc)];
+    T.ne = a.eta[b.off(clampi(s + 3, 0, nr - 1), jcc, kcc)];
+    load_ring(T, a, s + 2);
 }
 
 // 7-point star max of eta at (p, cj, ck) with in-plane values from LDS
@@ -210,6 +298,7 @@ SSPT_DH inline double star(const Lds &L, int ty, int tx, double em, double ep)
 }
 
 // ---- phase 2: etaSM(s+1); C(s) = P, tau_nn, DT at centres; tau_thph(s); VR face s
+template <int FL>
 SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
 {
     const BoxG &b = a.b;
@@ -236,26 +325,27 @@ SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
     const bool centre = T.j >= 0 && T.j < nt && T.k >= 0 && T.k < np;
     // C(s): compute_P! centre part
     if (s >= 0 && s <= nr - 1 && centre && T.tx < TK - 1 && T.ty < TJB - 1) {
-        const double R = t.Rc[s];
+        const double R = t.tr[s * NTR + TR_Rc], iR = t.tr[s * NTR + TR_invRc];
         const double dVRr = (T.vr1 - T.vr0) * a._dr;
-        const double dVTt = (t.invRc[s] * (L.VT[T.ty + 1][T.tx] - T.vt0)) * a._dth;
-        const double dVPp = (t.rs_c[s * (nt + 1) + T.j] * (L.VP[T.ty][T.tx + 1] - T.vp0)) * a._dph;
+        const double dVTt = (iR * (L.VT[T.ty + 1][T.tx] - T.vt0)) * a._dth;
+        const double dVPp = (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c] * (L.VP[T.ty][T.tx + 1] - T.vp0)) * a._dph;
         const double avVR = (T.vr0 + T.vr1) * 0.5;
         const double avVT = (T.vt0 + L.VT[T.ty + 1][T.tx]) * 0.5;
-        const double ct = t.cotc[T.j];
-        const double vr_R = avVR / R, vt_R = avVT / R;
-        const double divV = (((dVRr + dVTt) + dVPp) + (2.0 * avVR) / R) + vt_R * ct;
+        const double ct = t.tt[T.j * NTT + TT_cotc];
+        const double vr_R = div_rcp(avVR, R, iR), vt_R = div_rcp(avVT, R, iR);
+        const double divV = (((dVRr + dVTt) + dVPp) + div_rcp(2.0 * avVR, R, iR)) + vt_R * ct;
         const double third = 1.0 / 3.0;
         const double Drr = dVRr - third * divV;
         const double Dtt = (dVTt + vr_R) - third * divV;
         const double Dpp = ((dVPp + vr_R) + vt_R * ct) - third * divV;
         const double KB = a.kb * T.sm0, GS = a.gs * T.sm0;
         const double den = 1.0 + GS / T.e0;
+        const double iden = 1.0 / den;
         const double nP = T.oP - KB * divV;
-        T.nRR = (T.oRR + (GS * 2.0) * Drr) / den;
-        T.nTT = (T.oTT + (GS * 2.0) * Dtt) / den;
-        T.nPP = (T.oPP + (GS * 2.0) * Dpp) / den;
-        T.nDT = a.vpdt2 / (KB + GS / den);
+        T.nRR = div_rcp(T.oRR + (GS * 2.0) * Drr, den, iden);
+        T.nTT = div_rcp(T.oTT + (GS * 2.0) * Dtt, den, iden);
+        T.nPP = div_rcp(T.oPP + (GS * 2.0) * Dpp, den, iden);
+        T.nDT = a.vpdt2 / (KB + div_rcp(GS, den, iden));
         T.nSRR = -nP + T.nRR;
         T.nSTT = -nP + T.nTT;
         T.nSPP = -nP + T.nPP;
@@ -265,7 +355,7 @@ SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
             a.oRR[o] = T.nRR;
             a.oTT[o] = T.nTT;
             a.oPP[o] = T.nPP;
-            if (a.flags & F_LAST) {
+            if (FL & F_LAST) {
                 a.DT[o] = T.nDT;
                 a.SM[o] = T.sm0;
             }
@@ -285,8 +375,8 @@ SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
         const double dVP = T.vp0 - L.VP[T.ty - 1][T.tx];
         const double dVT = T.vt0 - L.VT[T.ty][T.tx - 1];
         const double avVP = (L.VP[T.ty - 1][T.tx] + T.vp0) * 0.5;
-        const double D = ((t.invRc4[s] * dVP) * a._dth + (t.rs_c4[s * (nt + 1) + T.j] * dVT) * a._dph) -
-                         (avVP / t.Rc4[s]) * t.cotf4[T.j];
+        const double D = ((t.tr[s * NTR + TR_invRc4] * dVP) * a._dth + (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c4] * dVT) * a._dph) -
+                         div_rcp(avVP, t.tr[s * NTR + TR_Rc4], t.tr[s * NTR + TR_invRc4]) * t.tt[T.j * NTT + TT_cotf4];
         T.ntp = (T.oTPold + T.gtp * D) / (1.0 + T.gtp / T.etp);
         if (wr && T.own) a.oTP[b.off(s, T.j, T.k)] = T.ntp;
     }
@@ -294,35 +384,38 @@ SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
 
     // VR at r-face s (compute_V!): needs C(s-1), C(s), tau_Rth/tau_Rph of face s
     if (wr && T.own && s >= 1 && s <= nr - 1 && T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 2) {
-        const double Rx = t.Rf[s];
+        const double Rx = t.tr[s * NTR + TR_Rf], iRx = t.tr[s * NTR + TR_invRf];
         const double sRR = (T.cRR + T.nSRR) * 0.5;
         const double sTT = (T.cTT + T.nSTT) * 0.5;
         const double sPP = (T.cPP + T.nSPP) * 0.5;
         const double aRT = (T.rt0 + T.rt0n) * 0.5;
         const double rg = (T.crho * a.g + T.orho * a.g) * 0.5;
-        const double dV = ((((((T.nSRR - T.cRR) * a._dr + (t.invRf[s] * (T.rt0n - T.rt0)) * a._dth) +
-                              (t.rs_f[s * (nt + 1) + T.j] * (T.rp0n - T.rp0)) * a._dph) +
-                             (2.0 * sRR) / Rx) -
-                            (sTT + sPP) / Rx) +
-                           (aRT / Rx) * t.cotc2[T.j]) -
+        const double dV = ((((((T.nSRR - T.cRR) * a._dr + (iRx * (T.rt0n - T.rt0)) * a._dth) +
+                              (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_f] * (T.rp0n - T.rp0)) * a._dph) +
+                             div_rcp(2.0 * sRR, Rx, iRx)) -
+                            div_rcp(sTT + sPP, Rx, iRx)) +
+                           div_rcp(aRT, Rx, iRx) * t.tt[T.j * NTT + TT_cotc2]) -
                           rg;
         const double dt = (T.cDT + T.nDT) * 0.5;
         const double v = T.vr0 + dV * dt;
         const long long o = b.off(s, T.j, T.k);
         a.oVR[o] = v;
-        if (a.flags & F_CHECK) {
+        if (FL & F_CHECK) {
             const double d = v - T.vr0;
             a.VRe[o] = d;
             const unsigned long long ud = absbits(d), uv = absbits(v);
             T.mdv[0] = ud > T.mdv[0] ? ud : T.mdv[0];
             T.mv[0] = uv > T.mv[0] ? uv : T.mv[0];
-        } else if (a.flags & F_LAST) {
+        } else if (FL & F_LAST) {
             a.VRe[o] = T.vr0;
         }
     }
+    T.crho = T.orho;
+    load_centre_old(T, a, s + 1);
 }
 
 // ---- phase 3: tau_Rth / tau_Rph on r-face s+1; 4-point sums of plane s+1
+template <int FL>
 SSPT_DH inline void phase3(Thread &T, Lds &L, const Args &a, int s, bool wr)
 {
     const BoxG &b = a.b;
@@ -349,57 +442,59 @@ SSPT_DH inline void phase3(Thread &T, Lds &L, const Args &a, int s, bool wr)
     T.nrt = 0.0;
     T.nrp = 0.0;
     if (f >= 1 && f <= nr - 1 && inner) {
-        const double Re = t.Re[f];
+        const double Re = t.tr[f * NTR + TR_Re], iRe = t.tr[f * NTR + TR_invRe];
         if (T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 2) {
             const double G = ((T.pG_rt + gs * L.SM[T.ty - 1][T.tx]) + gs * T.sm1) * 0.25;
             const double E = ((T.pE_rt + L.ETA[T.ty][T.tx + 1]) + T.e1) * 0.25;
-            const double D = ((T.vt1 - T.vt0) * a._dr + (t.invRe[f] * (T.vr1 - L.VR[T.ty - 1][T.tx])) * a._dth) -
-                             ((T.vt0 + T.vt1) * 0.5) / Re;
+            const double D = ((T.vt1 - T.vt0) * a._dr + (iRe * (T.vr1 - L.VR[T.ty - 1][T.tx])) * a._dth) -
+                             div_rcp((T.vt0 + T.vt1) * 0.5, Re, iRe);
             T.nrt = (T.oRTold + G * D) / (1.0 + G / E);
             if (wr && T.own) a.oRT[b.off(f, T.j, T.k)] = T.nrt;
         }
         if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
             const double G = ((T.pG_rp + gs * L.SM[T.ty][T.tx - 1]) + gs * T.sm1) * 0.25;
             const double E = ((T.pE_rp + L.ETA[T.ty + 1][T.tx]) + T.e1) * 0.25;
-            const double D = ((T.vp1 - T.vp0) * a._dr + (t.rs_e[f * (nt + 1) + T.j] * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
-                             ((T.vp0 + T.vp1) * 0.5) / Re;
+            const double D = ((T.vp1 - T.vp0) * a._dr + (t.t2[(f * (nt + 1) + T.j) * NT2 + T2_e] * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
+                             div_rcp((T.vp0 + T.vp1) * 0.5, Re, iRe);
             T.nrp = (T.oRPold + G * D) / (1.0 + G / E);
             if (wr && T.own) a.oRP[b.off(f, T.j, T.k)] = T.nrp;
         }
     }
     L.RT[T.ty][T.tx] = T.nrt;
     L.RP[T.ty][T.tx] = T.nrp;
+    load_face_old(T, a, s + 1);
 }
 
 // ---- phase 4: Vth(s), Vph(s), tII(s) -> eta(s); shift carries
+template <int FL>
 SSPT_DH inline void phase4(Thread &T, Lds &L, const Args &a, int s, bool wr)
 {
     const BoxG &b = a.b;
     const TabsG &t = a.t;
     const int nr = b.nr, nt = b.nt, np = b.np;
     if (wr && T.own && s >= 1 && s <= nr - 2) {
-        const double Ry = t.Rc2[s];
+        const double Ry = t.tr[s * NTR + TR_Rc2], iRy = t.tr[s * NTR + TR_invRc2];
         // Vth at (s, j, k)
         if (T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 2) {
             const double sTTm = L.STT[T.ty - 1][T.tx], sPPm = L.SPP[T.ty - 1][T.tx];
             const double sTT = (sTTm + T.nSTT) * 0.5, sPP = (sPPm + T.nSPP) * 0.5;
             const double aRT = (T.rt0 + T.nrt) * 0.5;
             const double tpK = L.TP[T.ty][T.tx + 1];
-            const double dV = ((((T.nrt - T.rt0) * a._dr + (t.invRc2[s] * (T.nSTT - sTTm)) * a._dth) +
-                                (t.rs_y[s * (nt + 1) + T.j] * (tpK - T.ntp)) * a._dph) +
-                               (3.0 * aRT) / Ry) +
-                              ((sTT - sPP) / Ry) * t.cotf2[T.j];
+            const double dV = ((((T.nrt - T.rt0) * a._dr + (iRy * (T.nSTT - sTTm)) * a._dth) +
+                                (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_y] * (tpK - T.ntp)) * a._dph) +
+                               div_rcp(3.0 * aRT, Ry, iRy)) +
+                              div_rcp(sTT - sPP, Ry, iRy) * t.tt[T.j * NTT + TT_cotf2];
             const double dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * 0.5;
             const double v = T.vt0 + dV * dt;
             const long long o = b.off(s, T.j, T.k);
             a.oVT[o] = v;
-            if (a.flags & F_CHECK) {
+            if (FL & F_CHECK) {
                 const double d = v - T.vt0;
                 a.VTe[o] = d;
                 const unsigned long long ud = absbits(d), uv = absbits(v);
                 T.mdv[1] = ud > T.mdv[1] ? ud : T.mdv[1];
                 T.mv[1] = uv > T.mv[1] ? uv : T.mv[1];
-            } else if (a.flags & F_LAST) {
+            } else if (FL & F_LAST) {
                 a.VTe[o] = T.vt0;
             }
         }
@@ -409,21 +504,21 @@ SSPT_DH inline void phase4(Thread &T, Lds &L, const Args &a, int s, bool wr)
             const double sPPm = L.SPP[T.ty][T.tx - 1];
             const double aRP = (T.rp0 + T.nrp) * 0.5;
             const double aTP = (T.ntp + tpJ) * 0.5;
-            const double dV = ((((T.nrp - T.rp0) * a._dr + (t.invRc2[s] * (tpJ - T.ntp)) * a._dth) +
-                                (t.rs_z[s * (nt + 1) + T.j] * (T.nSPP - sPPm)) * a._dph) +
-                               (3.0 * aRP) / Ry) +
-                              ((2.0 * aTP) / Ry) * t.cotc2[T.j];
+            const double dV = ((((T.nrp - T.rp0) * a._dr + (iRy * (tpJ - T.ntp)) * a._dth) +
+                                (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_z] * (T.nSPP - sPPm)) * a._dph) +
+                               div_rcp(3.0 * aRP, Ry, iRy)) +
+                              div_rcp(2.0 * aTP, Ry, iRy) * t.tt[T.j * NTT + TT_cotc2];
             const double dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * 0.5;
             const double v = T.vp0 + dV * dt;
             const long long o = b.off(s, T.j, T.k);
             a.oVP[o] = v;
-            if (a.flags & F_CHECK) {
+            if (FL & F_CHECK) {
                 const double d = v - T.vp0;
                 a.VPe[o] = d;
                 const unsigned long long ud = absbits(d), uv = absbits(v);
                 T.mdv[2] = ud > T.mdv[2] ? ud : T.mdv[2];
                 T.mv[2] = uv > T.mv[2] ? uv : T.mv[2];
-            } else if (a.flags & F_LAST) {
+            } else if (FL & F_LAST) {
                 a.VPe[o] = T.vp0;
             }
         }
@@ -435,11 +530,11 @@ SSPT_DH inline void phase4(Thread &T, Lds &L, const Args &a, int s, bool wr)
             const double tii =
                 std::sqrt((((1.0 / 2.0) * ((T.nRR * T.nRR + T.nTT * T.nTT) + T.nPP * T.nPP) + x * x) + y * y) + z * z);
             double pl;
-            const double e = power_law_pt(tii, T.oini, T.oPL, &pl, a);
+            const double e = power_law_pt(tii, T.oini, T.oPL, &pl, a, FL);
             const long long o = b.off(s, T.j, T.k);
             a.oEta[o] = e;
-            if (a.flags & F_RELAX) a.oPL[o] = pl;
-            if (a.flags & F_LAST) a.TII[o] = tii;
+            if (FL & F_RELAX) a.oPL[o] = pl;
+            if (FL & F_LAST) a.TII[o] = tii;
         }
     }
     // carries for step s+1
@@ -451,7 +546,6 @@ SSPT_DH inline void phase4(Thread &T, Lds &L, const Args &a, int s, bool wr)
     T.cTT = T.nSTT;
     T.cPP = T.nSPP;
     T.cDT = T.nDT;
-    T.crho = T.orho;
     T.e0 = T.e1;
     T.e1 = T.e2;
     T.sm0 = T.sm1;
@@ -461,6 +555,7 @@ SSPT_DH inline void phase4(Thread &T, Lds &L, const Args &a, int s, bool wr)
     T.pE_rp = T.pE_rp2;
     T.gtp = T.gtp2;
     T.etp = T.etp2;
+    load_visc_old<FL>(T, a, s + 1);
 }
 
 // ---------------------------------------------------------------- boundary pass
@@ -589,7 +684,7 @@ SSPT_DH inline void boundary_eta_point(const Args &a, int i, int j, int k)
     const long long o = b.off(i, j, k);
     double pl;
     const double oldpl = (a.flags & F_RELAX) ? a.ePL[o] : 0.0;
-    const double e = power_law_pt(tii, a.ini[o], oldpl, &pl, a);
+    const double e = power_law_pt(tii, a.ini[o], oldpl, &pl, a, a.flags);
     a.oEta[o] = e;
     if (a.flags & F_RELAX) a.oPL[o] = pl;
     if (a.flags & F_LAST) a.TII[o] = tii;

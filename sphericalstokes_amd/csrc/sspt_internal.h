@@ -79,6 +79,8 @@ struct ss_ctx {
     bool alt_ready = false;
     double *tab_mem = nullptr;
     sspt::Tables tab{};
+    double *ftab_mem = nullptr;     // interleaved tables of the fused kernel
+    const double *ftr = nullptr, *ftt = nullptr, *ft2 = nullptr;
     bool grid_set = false;
     double *staging = nullptr;      // upload/download staging (largest field)
     size_t staging_elems = 0;

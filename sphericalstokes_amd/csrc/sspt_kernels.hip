@@ -7,6 +7,7 @@
 // except power_law's pow() (device libm, within an ulp).
 #include "sspt_internal.h"
 #include "sspt_launch.h"
+#include "sspt_fused_body.h"
 #include <cmath>
 
 namespace sspt {
@@ -163,7 +164,7 @@ __global__ void __launch_bounds__(256) k_compute_TII(Box b, double *__restrict__
 __device__ __forceinline__ double power_law_point(double tii, double ini, double *ePL, double tC, double ex,
                                                   double relax, bool do_relax, double emin, double emax)
 {
-    double pl = ini * pow(tii / tC, ex);
+    double pl = ini * fused::pow_ex(tii / tC, ex);
     if (do_relax) {
         pl = exp(log(pl) * relax + log(*ePL) * (1.0 - relax));
         *ePL = pl;

@@ -81,4 +81,33 @@ inline void build_tables(const BoxT &b, const double *R, const double *th, const
         }
 }
 
+// Interleaved records for the fused kernel: per r index 16 doubles
+// (Rc invRc Rf invRf Re invRe Rc4 invRc4 Rc2 invRc2, padding), per th index 4
+// (cotc cotf4 cotc2 cotf2), per (i,j) 8 ((1/R)/sin(th) for the centre, the
+// tau_Rph edge, the tau_thph edge and the VR, Vth, Vph faces, padding).  The
+// values are copied from build_tables' arrays, bit for bit.
+constexpr int FT_NTR = 16, FT_NTT = 4, FT_NT2 = 8;
+
+template <class BoxT> inline size_t fused_tables_elems(const BoxT &b)
+{
+    const size_t n1 = b.nr + 1, n2 = b.nt + 1;
+    return FT_NTR * n1 + FT_NTT * n2 + FT_NT2 * n1 * n2;
+}
+
+template <class BoxT>
+inline void build_fused_tables(const BoxT &b, const std::vector<double> &h, std::vector<double> &f)
+{
+    const size_t n1 = b.nr + 1, n2 = b.nt + 1;
+    f.assign(fused_tables_elems(b), 0.0);
+    const double *src = h.data();
+    for (int q = 0; q < 10; ++q, src += n1)
+        for (size_t i = 0; i < n1; ++i) f[i * FT_NTR + q] = src[i];
+    double *tt = f.data() + FT_NTR * n1;
+    for (int q = 0; q < 4; ++q, src += n2)
+        for (size_t j = 0; j < n2; ++j) tt[j * FT_NTT + q] = src[j];
+    double *t2 = tt + FT_NTT * n2;
+    for (int q = 0; q < 6; ++q, src += n1 * n2)
+        for (size_t ij = 0; ij < n1 * n2; ++ij) t2[ij * FT_NT2 + q] = src[ij];
+}
+
 } // namespace sspt

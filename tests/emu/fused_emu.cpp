@@ -7,6 +7,7 @@
 // fused schedule (tiling, halos, r-march, carries, boundary pass) against the
 // CPU oracle without a GPU.  It is never part of the product library.
 #define SSPT_DH
+#define SSPT_UNIFORM(x) (x)
 #include "../../sphericalstokes_amd/csrc/sspt_fused_body.h"
 #include "../../sphericalstokes_amd/csrc/sspt_tables.h"
 #include "../../include/sspt.h"
@@ -26,7 +27,8 @@ struct HostBox {
     long long sj, si;
 };
 
-void run_main(const Args &a, int rchunk, unsigned long long red[6])
+template <int FL>
+void run_main_t(const Args &a, int rchunk, unsigned long long red[6])
 {
     const int nx = (a.b.np + OK_ - 1) / OK_;
     const int ny = (a.b.nt + OJ - 1) / OJ;
@@ -44,14 +46,14 @@ void run_main(const Args &a, int rchunk, unsigned long long red[6])
                     for (int tx = 0; tx < TK; ++tx) {
                         Thread &T = th[ty * TK + tx];
                         thread_init(T, a, tx, ty, kb, jb);
-                        thread_prologue(T, a, start);
+                        thread_prologue<FL>(T, a, start);
                     }
                 for (int s = start; s < s1; ++s) {
                     const bool wr = s >= s0;
-                    for (auto &T : th) phase1(T, *L, a, s);
-                    for (auto &T : th) phase2(T, *L, a, s, wr);
-                    for (auto &T : th) phase3(T, *L, a, s, wr);
-                    for (auto &T : th) phase4(T, *L, a, s, wr);
+                    for (auto &T : th) phase1<FL>(T, *L, a, s);
+                    for (auto &T : th) phase2<FL>(T, *L, a, s, wr);
+                    for (auto &T : th) phase3<FL>(T, *L, a, s, wr);
+                    for (auto &T : th) phase4<FL>(T, *L, a, s, wr);
                 }
                 for (auto &T : th)
                     for (int c = 0; c < 3; ++c) {
@@ -59,6 +61,17 @@ void run_main(const Args &a, int rchunk, unsigned long long red[6])
                         red[3 + c] = std::max(red[3 + c], T.mv[c]);
                     }
             }
+}
+
+void run_main(const Args &a, int rchunk, unsigned long long red[6])
+{
+    switch (a.flags) {
+#define SSPT_FL(v) case v: run_main_t<v>(a, rchunk, red); break;
+    SSPT_FL(0) SSPT_FL(1) SSPT_FL(3) SSPT_FL(4) SSPT_FL(5) SSPT_FL(7)
+    SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15)
+#undef SSPT_FL
+    default: break;
+    }
 }
 
 void run_boundary(const Args &a, unsigned long long red[6])
@@ -114,19 +127,17 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
                          unsigned long long red[6])
 {
     HostBox hb{nr, nt, np, pitch, (long long)(nt + 1) * pitch};
-    std::vector<double> tab;
+    std::vector<double> tab, ftab;
     sspt::build_tables(hb, R1, th1, ph1, tab);
+    sspt::build_fused_tables(hb, tab, ftab);
     const size_t n1 = nr + 1, n2 = nt + 1;
-    const double *q = tab.data();
-    auto take = [&](size_t n) { const double *r = q; q += n; return r; };
     TabsG t;
-    t.Rc = take(n1); t.invRc = take(n1); t.Rf = take(n1); t.invRf = take(n1);
-    t.Re = take(n1); t.invRe = take(n1); t.Rc4 = take(n1); t.invRc4 = take(n1);
-    t.Rc2 = take(n1); t.invRc2 = take(n1);
-    t.cotc = take(n2); t.cotf4 = take(n2); t.cotc2 = take(n2); t.cotf2 = take(n2);
-    t.rs_c = take(n1 * n2); t.rs_e = take(n1 * n2); t.rs_c4 = take(n1 * n2);
-    t.rs_f = take(n1 * n2); t.rs_y = take(n1 * n2); t.rs_z = take(n1 * n2);
-    t.th = take(n2); t.ph = take(np + 1);
+    t.tr = ftab.data();
+    t.tt = t.tr + sspt::FT_NTR * n1;
+    t.t2 = t.tt + sspt::FT_NTT * n2;
+    // th and ph follow the 20 separate tables in build_tables' layout
+    t.th = tab.data() + 10 * n1 + 4 * n2 + 6 * n1 * n2;
+    t.ph = t.th + n2;
     static constexpr double PI = 3.14159265358979323846;
     for (int c = 0; c < 6; ++c) red[c] = 0ull;
     double **in = cur, **out = alt;
@@ -153,7 +164,7 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
         a.g = p->g; a.tC = p->tau_C; a.ex = 1.0 - p->n_exp; a.relax = p->relax;
         a.emin = p->eta_min; a.emax = p->eta_max;
         a.flags = (last ? F_LAST : 0) | (check ? F_CHECK : 0) | (p->relax != 1.0 ? F_RELAX : 0) |
-                  (p->eta_min > 0.0 ? F_CLAMP_LO : 0) | (p->eta_max < INFINITY ? F_CLAMP_HI : 0);
+                  ((p->eta_min > 0.0 || p->eta_max < INFINITY) ? F_CLAMP : 0);
         a.bc_mode = p->bc_mode; a.r = p->r; a.sh = p->sh;
         a.rchunk = rchunk > 0 ? rchunk : nr;
         run_main(a, a.rchunk, red);

@@ -2,8 +2,10 @@
 
 Per-kernel entry points must match the oracle bit for bit (-ffp-contract=off,
 tables built with the reference association, correctly rounded IEEE division
-and sqrt); the only exception is power_law's pow(), whose device libm may
-differ by an ulp, so kernels that run it are compared within 4 ulp / 1e-14.
+and sqrt, divisions by metric coefficients through a correctly rounded
+reciprocal and one FMA correction, which returns the IEEE quotient); the only
+exception is x^(1-n_exp) in power_law (products and one reciprocal for integer
+n_exp, device pow otherwise, against the host's pow), compared within 4 ulp.
 The fused loop must match the oracle's iteration sequence bit for bit when
 pow is not involved (n_exp = 1 makes the exponent 0) and to rel-L2 <= 1e-10
 otherwise, the FP64 tolerance of the drop-in.
