@@ -289,12 +289,15 @@ SSPT_DH inline void phase1(Thread &T, Lds &L, const Args &a, int s)
 }
 
 // 7-point star max of eta at (p, cj, ck) with in-plane values from LDS
-// (@maxloc expansion, SphericalStokes_mxpu.jl:295-297).
+// (@maxloc expansion, SphericalStokes_mxpu.jl:295-297).  The hardware max
+// (fmax) equals Julia's max for the positive, finite viscosities of a valid
+// state; it does not propagate a NaN viscosity into etaSM (a NaN still
+// reaches V through eta itself, so the NaN exit test still fires).
 SSPT_DH inline double star(const Lds &L, int ty, int tx, double em, double ep)
 {
     const double c = L.ETA[ty + 1][tx + 1];
-    return jmax(jmax(jmax(jmax(em, ep), c), jmax(L.ETA[ty][tx + 1], L.ETA[ty + 2][tx + 1])),
-                jmax(L.ETA[ty + 1][tx], L.ETA[ty + 1][tx + 2]));
+    return std::fmax(std::fmax(std::fmax(std::fmax(em, ep), c), std::fmax(L.ETA[ty][tx + 1], L.ETA[ty + 2][tx + 1])),
+                     std::fmax(L.ETA[ty + 1][tx], L.ETA[ty + 1][tx + 2]));
 }
 
 // ---- phase 2: etaSM(s+1); C(s) = P, tau_nn, DT at centres; tau_thph(s); VR face s
