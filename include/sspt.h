@@ -136,6 +136,41 @@ int ss_pt_solve(ss_ctx *ctx, const ss_params *p, int maxiter, int nout, double e
 int ss_pt_solve_unfused(ss_ctx *ctx, const ss_params *p, int maxiter, int nout,
                         double eps, ss_result *out);
 
+/* ---- true residual, time-to-tolerance and the mixed-precision solve ---- */
+/* Residual of the current state with tau replaced by the stress consistent
+ * with V and eta: out[1] = max|divV|*lr/max|V|, out[2] = max|div(sigma) -
+ * rho g| / max|rho g| (or / (max|sigma|/lr) without a body force),
+ * out[0] = max(out[1], out[2]).  The state is not modified. */
+int ss_residual(ss_ctx *ctx, const ss_params *p, double out[3]);
+/* Plain FP64 fused PT iterations until ss_residual <= tol, checked every
+ * ncheck iterations.  out->err = last residual, out->iter_end = iterations
+ * at convergence (0 if not), out->seconds = wall time.  No reference
+ * counterpart: the reference stops on the velocity change (ss_pt_solve). */
+int ss_pt_solve_residual(ss_ctx *ctx, const ss_params *p, int maxiter, int ncheck, double tol, ss_result *out);
+
+#define SS_MIXED_MAX_OUTER 64
+typedef struct ss_mixed_opts {
+    int max_outer;     /* FP64 outer (defect-correction) steps, <= SS_MIXED_MAX_OUTER */
+    int max_inner;     /* FP32 PT sweeps per outer step */
+    int nout_inner;    /* sweeps between convergence checks of the correction */
+    double tol_inner;  /* stop an outer step's sweeps when max|dd|/max|d| < tol_inner */
+    double tol;        /* stop when ss_residual of the FP64 iterate <= tol */
+} ss_mixed_opts;
+typedef struct ss_mixed_result {
+    int outer, inner_total, converged;
+    int inner_hist[SS_MIXED_MAX_OUTER];
+    double r_hist[SS_MIXED_MAX_OUTER + 1];   /* residual before each outer step (and final) */
+    double err_inner_hist[SS_MIXED_MAX_OUTER];
+    double r, seconds;
+} ss_mixed_result;
+/* FP32 PT sweeps inside an FP64 defect-correction loop (no reference
+ * counterpart).  Each outer step resets tau to the stress consistent with V
+ * and eta, evaluates the FP64 continuity and momentum residuals, runs FP32
+ * fused sweeps on the correction (frozen eta, homogeneous BCs, residual
+ * forcing), adds the correction in FP64 and, for a power law, updates eta.
+ * Converges to the FP64 solution of the same equations. */
+int ss_pt_solve_mixed(ss_ctx *ctx, const ss_params *p, const ss_mixed_opts *o, ss_mixed_result *res);
+
 /* ---- timing hooks for the benchmark (no reference counterpart) ---- */
 /* Average device time (ms) per iteration of the last ss_pt_iterate, from HIP
  * events on the context's stream. */

@@ -38,6 +38,20 @@ class ss_result(C.Structure):
                 ("err_comp", C.c_double * 3), ("seconds", C.c_double)]
 
 
+MIXED_MAX_OUTER = 64
+
+
+class ss_mixed_opts(C.Structure):
+    _fields_ = [("max_outer", C.c_int), ("max_inner", C.c_int), ("nout_inner", C.c_int),
+                ("tol_inner", C.c_double), ("tol", C.c_double)]
+
+
+class ss_mixed_result(C.Structure):
+    _fields_ = [("outer", C.c_int), ("inner_total", C.c_int), ("converged", C.c_int),
+                ("inner_hist", C.c_int * MIXED_MAX_OUTER), ("r_hist", C.c_double * (MIXED_MAX_OUTER + 1)),
+                ("err_inner_hist", C.c_double * MIXED_MAX_OUTER), ("r", C.c_double), ("seconds", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/sspt.h
 P = C.c_void_p
 D = C.c_double
@@ -70,6 +84,9 @@ SIGNATURES = {
     "ss_pt_iterate": (I, [P, C.POINTER(ss_params), I]),
     "ss_pt_solve": (I, [P, C.POINTER(ss_params), I, I, D, C.POINTER(ss_result)]),
     "ss_pt_solve_unfused": (I, [P, C.POINTER(ss_params), I, I, D, C.POINTER(ss_result)]),
+    "ss_residual": (I, [P, C.POINTER(ss_params), DP]),
+    "ss_pt_solve_residual": (I, [P, C.POINTER(ss_params), I, I, D, C.POINTER(ss_result)]),
+    "ss_pt_solve_mixed": (I, [P, C.POINTER(ss_params), C.POINTER(ss_mixed_opts), C.POINTER(ss_mixed_result)]),
     "ss_last_kernel_ms": (I, [P, DP]),
     "ss_pt_iterate_timed": (I, [P, C.POINTER(ss_params), I, DP, DP]),
 }

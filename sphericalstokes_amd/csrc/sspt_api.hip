@@ -173,6 +173,15 @@ int ss_create(int nr, int nth, int nph, int device, ss_ctx **out)
     c->ftr = c->ftab_mem;
     c->ftt = c->ftr + (size_t)FT_NTR * (nr + 1);
     c->ft2 = c->ftt + (size_t)FT_NTT * (nth + 1);
+    {
+        const size_t n32 = fused_tables_elems(b) + (nth + 1) + (nph + 1);
+        if (hipMalloc(&c->ftab32_mem, n32 * sizeof(float)) != hipSuccess) { ss_destroy(c); return SS_ERR_NOMEM; }
+        c->ftr32 = c->ftab32_mem;
+        c->ftt32 = c->ftr32 + (size_t)FT_NTR * (nr + 1);
+        c->ft232 = c->ftt32 + (size_t)FT_NTT * (nth + 1);
+        c->th32 = c->ft232 + (size_t)FT_NT2 * (nr + 1) * (nth + 1);
+        c->ph32 = c->th32 + (nth + 1);
+    }
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
     rc = sync(c, "ss_create");
@@ -193,6 +202,11 @@ int ss_destroy(ss_ctx *c)
     if (c->red_host) (void)hipHostFree(c->red_host);
     if (c->tab_mem) (void)hipFree(c->tab_mem);
     if (c->ftab_mem) (void)hipFree(c->ftab_mem);
+    if (c->ftab32_mem) (void)hipFree(c->ftab32_mem);
+    for (double *p : c->tau_scr)
+        if (p) (void)hipFree(p);
+    for (float *p : c->f32)
+        if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -226,6 +240,11 @@ int ss_set_grid(ss_ctx *c, const double *R1, const double *th1, const double *ph
     std::vector<double> hf;
     build_fused_tables(c->box, h, hf);
     HIP_TRY(c, hipMemcpyAsync(c->ftab_mem, hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // FP32 copies for the correction sweeps (tables, then th and ph)
+    std::vector<float> h32(hf.begin(), hf.end());
+    const double *thp = h.data() + (h.size() - (c->box.nt + 1) - (c->box.np + 1));
+    h32.insert(h32.end(), thp, thp + (c->box.nt + 1) + (c->box.np + 1));
+    HIP_TRY(c, hipMemcpyAsync(c->ftab32_mem, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     int rc = sync(c, "ss_set_grid");
     if (rc == SS_OK) c->grid_set = true;
     return rc;

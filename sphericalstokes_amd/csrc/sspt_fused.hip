@@ -20,7 +20,8 @@ __device__ __forceinline__ unsigned long long wave_max(unsigned long long v)
     return v;
 }
 
-__device__ __forceinline__ void flush_max(const Args &a, const unsigned long long *mdv, const unsigned long long *mv,
+template <typename R>
+__device__ __forceinline__ void flush_max(const Args<R> &a, const unsigned long long *mdv, const unsigned long long *mv,
                                           int lane)
 {
 #pragma unroll
@@ -48,37 +49,38 @@ __device__ __forceinline__ void tile_of(int nx, int ny, int nz, int &bx, int &by
     bz = t / (nx * ny);
 }
 
-template <int FL>
-__global__ void __launch_bounds__(TK * TJB) k_fused_main(Args a, int nx, int ny, int nz)
+template <typename R, int FL>
+__global__ void __launch_bounds__(TK * TJB) k_fused_main(Args<R> a, int nx, int ny, int nz)
 {
-    __shared__ Lds L;
+    __shared__ Lds<R> L;
     int bx, by, bz;
     tile_of(nx, ny, nz, bx, by, bz);
     if (bz >= nz) return; // padding slots of the remap (uniform per block)
     const int kb = bx * OK_, jb = by * OJ;
     const int s0 = bz * a.rchunk, s1 = min(s0 + a.rchunk, a.b.nr);
     const int start = max(s0 - 2, -1);
-    Thread T;
+    Thread<R> T;
     thread_init(T, a, threadIdx.x, threadIdx.y, kb, jb);
-    thread_prologue<FL>(T, a, start);
+    thread_prologue<R, FL>(T, a, start);
     for (int s = start; s < s1; ++s) {
         const bool wr = s >= s0;
-        phase1<FL>(T, L, a, s);
+        phase1<R, FL>(T, L, a, s);
         __syncthreads();
-        phase2<FL>(T, L, a, s, wr);
+        phase2<R, FL>(T, L, a, s, wr);
         __syncthreads();
-        phase3<FL>(T, L, a, s, wr);
+        phase3<R, FL>(T, L, a, s, wr);
         __syncthreads();
         // phase4 reads only STT/SPP/DT/TP/RT/RP; the next phase1 writes only
         // VR/VT/VP/ETA, so no barrier is needed between them.
-        phase4<FL>(T, L, a, s, wr);
+        phase4<R, FL>(T, L, a, s, wr);
     }
     if (FL & F_CHECK) flush_max(a, T.mdv, T.mv, threadIdx.x);
 }
 
 // Boundary pass.  blockIdx.z = family: 0 -> r faces over (j,k),
 // 1 -> th faces over (i,k), 2 -> phi faces over (i,j).
-__global__ void __launch_bounds__(256) k_fused_boundary(Args a)
+template <typename R>
+__global__ void __launch_bounds__(256) k_fused_boundary(Args<R> a)
 {
     const BoxG &b = a.b;
     const int u = blockIdx.x * blockDim.x + threadIdx.x; // fastest coordinate
@@ -105,7 +107,9 @@ __global__ void __launch_bounds__(256) k_fused_boundary(Args a)
             }
         }
     }
-    if (fam == 0) {
+    if (a.flags & F_CORR) {
+        // eta is frozen in a correction sweep
+    } else if (fam == 0) {
         if (u < np && v < nt) {
             boundary_eta_point(a, 0, v, u);
             boundary_eta_point(a, nr - 1, v, u);
@@ -130,16 +134,16 @@ __global__ void k_zero_red(unsigned long long *p)
     if (threadIdx.x < 6) p[threadIdx.x] = 0ull;
 }
 
-static Args make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out, const ss_params &p, bool last, bool check)
+static Args<double> make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out, const ss_params &p, bool last, bool check)
 {
     static constexpr double PI = 3.14159265358979323846;
-    Args a;
+    Args<double> a{};
     a.b.nr = c->box.nr;
     a.b.nt = c->box.nt;
     a.b.np = c->box.np;
     a.b.sj = c->box.sj;
     a.b.si = c->box.si;
-    a.t = TabsG{c->ftr, c->ftt, c->ft2, c->tab.th, c->tab.ph};
+    a.t = TabsG<double>{c->ftr, c->ftt, c->ft2, c->tab.th, c->tab.ph};
     a.VR = in.f[SS_VR];
     a.VT = in.f[SS_VTH];
     a.VP = in.f[SS_VPH];
@@ -207,29 +211,47 @@ static int pick_rchunk(int nr, int tiles_xy)
     return (nr + nz - 1) / nz;
 }
 
-void launch_fused_iteration(ss_ctx *c, const FieldSet &in, const FieldSet &out, const ss_params &p, bool last,
-                            bool check, hipEvent_t ev_main0, hipEvent_t ev_main1)
+// Launches one fused iteration (main kernel + boundary pass) for prepared
+// arguments; the flag combination selects the kernel instantiation.
+template <typename R>
+void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_main1)
 {
-    Args a = make_args(c, in, out, p, last, check);
     const int nx = (a.b.np + OK_ - 1) / OK_;
     const int ny = (a.b.nt + OJ - 1) / OJ;
     a.rchunk = pick_rchunk(a.b.nr, nx * ny);
     const int nz = (a.b.nr + a.rchunk - 1) / a.rchunk;
-    if (check) hipLaunchKernelGGL(k_zero_red, dim3(1), dim3(64), 0, c->stream, c->red);
+    if (a.flags & F_CHECK) hipLaunchKernelGGL(k_zero_red, dim3(1), dim3(64), 0, c->stream, a.red);
     const int T = nx * ny * nz;
     const int grid = ((T + 7) / 8) * 8;
     if (ev_main0) (void)hipEventRecord(ev_main0, c->stream);
     const dim3 G(grid), B(TK, TJB);
-    switch (a.flags) {
-#define SSPT_FL(v) case v: hipLaunchKernelGGL(k_fused_main<v>, G, B, 0, c->stream, a, nx, ny, nz); break;
-    SSPT_FL(0) SSPT_FL(1) SSPT_FL(3) SSPT_FL(4) SSPT_FL(5) SSPT_FL(7)
-    SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15)
+    if constexpr (sizeof(R) == 8) {
+        switch (a.flags) {
+#define SSPT_FL(v) case v: hipLaunchKernelGGL((k_fused_main<R, v>), G, B, 0, c->stream, a, nx, ny, nz); break;
+        SSPT_FL(0) SSPT_FL(1) SSPT_FL(3) SSPT_FL(4) SSPT_FL(5) SSPT_FL(7)
+        SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15)
 #undef SSPT_FL
-    default: break; // CHECK without LAST is never requested
+        default: break; // CHECK without LAST is never requested in FP64
+        }
+    } else {
+        switch (a.flags) {
+#define SSPT_FL(v) case v: hipLaunchKernelGGL((k_fused_main<R, v>), G, B, 0, c->stream, a, nx, ny, nz); break;
+        SSPT_FL(16) SSPT_FL(18)
+#undef SSPT_FL
+        default: break; // FP32 runs correction sweeps only
+        }
     }
     if (ev_main1) (void)hipEventRecord(ev_main1, c->stream);
     const int m = std::max(std::max(a.b.nr, a.b.nt), a.b.np) + 1;
-    hipLaunchKernelGGL(k_fused_boundary, dim3((m + 63) / 64, (m + 3) / 4, 3), dim3(64, 4), 0, c->stream, a);
+    hipLaunchKernelGGL((k_fused_boundary<R>), dim3((m + 63) / 64, (m + 3) / 4, 3), dim3(64, 4), 0, c->stream, a);
+}
+
+template void launch_fused_args<float>(ss_ctx *, Args<float>, hipEvent_t, hipEvent_t);
+
+void launch_fused_iteration(ss_ctx *c, const FieldSet &in, const FieldSet &out, const ss_params &p, bool last,
+                            bool check, hipEvent_t ev_main0, hipEvent_t ev_main1)
+{
+    launch_fused_args<double>(c, make_args(c, in, out, p, last, check), ev_main0, ev_main1);
 }
 
 } // namespace sspt

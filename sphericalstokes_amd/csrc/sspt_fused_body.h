@@ -41,7 +41,10 @@ constexpr int OK_ = TK - 2;   // owned points per row
 constexpr int OJ = TJB - 2;   // owned rows
 constexpr double PI_ = 3.14159265358979323846;
 
-enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP = 8 };
+// F_CORR: correction sweep of the FP32/FP64 defect-correction solve (forcing
+// arrays instead of rho*g and the continuity right-hand side, frozen eta, no
+// power law; homogeneous BCs come from sh = 0).
+enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP = 8, F_CORR = 16 };
 
 struct BoxG {
     int nr, nt, np;
@@ -56,45 +59,48 @@ enum { TR_Rc, TR_invRc, TR_Rf, TR_invRf, TR_Re, TR_invRe, TR_Rc4, TR_invRc4, TR_
 enum { TT_cotc, TT_cotf4, TT_cotc2, TT_cotf2, NTT = 4 };
 enum { T2_c, T2_e, T2_c4, T2_f, T2_y, T2_z, NT2 = 8 };
 
-struct TabsG {
-    const double *tr; // [i*NTR + TR_*], i in [0, nr]
-    const double *tt; // [j*NTT + TT_*], j in [0, nt]
-    const double *t2; // [(i*(nt+1)+j)*NT2 + T2_*]
-    const double *th, *ph;
+template <typename R> struct TabsG {
+    const R *tr; // [i*NTR + TR_*], i in [0, nr]
+    const R *tt; // [j*NTT + TT_*], j in [0, nt]
+    const R *t2; // [(i*(nt+1)+j)*NT2 + T2_*]
+    const R *th, *ph;
 };
 
-struct Args {
+template <typename R> struct Args {
     BoxG b;
-    TabsG t;
+    TabsG<R> t;
     // old state
-    const double *VR, *VT, *VP, *eta, *P, *tRR, *tTT, *tPP, *tRT, *tRP, *tTP, *rho, *ini, *ePL;
+    const R *VR, *VT, *VP, *eta, *P, *tRR, *tTT, *tPP, *tRT, *tRP, *tTP, *rho, *ini, *ePL;
     // new state
-    double *oVR, *oVT, *oVP, *oEta, *oP, *oRR, *oTT, *oPP, *oRT, *oRP, *oTP, *oPL;
+    R *oVR, *oVT, *oVP, *oEta, *oP, *oRR, *oTT, *oPP, *oRT, *oRP, *oTP, *oPL;
     // auxiliary outputs (last iteration / convergence check)
-    double *SM, *DT, *TII, *VRe, *VTe, *VPe;
+    R *SM, *DT, *TII, *VRe, *VTe, *VPe;
+    // forcing of a correction sweep (F_CORR): divV of the FP64 iterate at
+    // centres and its momentum residual at faces
+    const R *gP, *fVR, *fVT, *fVP;
     unsigned long long *red; // [0..2] max|dV|, [3..5] max|V| as bit patterns
-    double _dr, _dth, _dph, kb, gs, vpdt2, g, tC, ex, relax, emin, emax;
+    R _dr, _dth, _dph, kb, gs, vpdt2, g, tC, ex, relax, emin, emax;
     int flags;
     int bc_mode;
-    double r, sh;
+    R r, sh;
     int rchunk; // r-steps per block along grid z
 };
 
-struct Lds {
-    double VR[TJB][TK], VT[TJB][TK], VP[TJB][TK];
-    double ETA[TJB + 2][TK + 2];
-    double SM[TJB][TK];
-    double STT[TJB][TK], SPP[TJB][TK], DT[TJB][TK];
-    double RT[TJB][TK], RP[TJB][TK], TP[TJB][TK];
+template <typename R> struct Lds {
+    R VR[TJB][TK], VT[TJB][TK], VP[TJB][TK];
+    R ETA[TJB + 2][TK + 2];
+    R SM[TJB][TK];
+    R STT[TJB][TK], SPP[TJB][TK], DT[TJB][TK];
+    R RT[TJB][TK], RP[TJB][TK], TP[TJB][TK];
 };
 
 // Julia Base.max/min for Float64
-SSPT_DH inline double jmax(double x, double y)
+template <typename R> SSPT_DH inline R jmax(R x, R y)
 {
     bool pick_y = (y > x) || (std::signbit(y) < std::signbit(x));
     return pick_y ? (std::isnan(x) ? x : y) : (std::isnan(y) ? y : x);
 }
-SSPT_DH inline double jmin(double x, double y)
+template <typename R> SSPT_DH inline R jmin(R x, R y)
 {
     bool pick_y = (y < x) || (std::signbit(y) > std::signbit(x));
     return pick_y ? (std::isnan(x) ? x : y) : (std::isnan(y) ? y : x);
@@ -106,42 +112,44 @@ SSPT_DH inline unsigned long long absbits(double x)
     c.d = std::fabs(x);
     return c.u;
 }
+// the max-reductions work on FP64 bit patterns in both precisions
+SSPT_DH inline unsigned long long absbits(float x) { return absbits((double)x); }
 
 // x / d for d with a correctly rounded reciprocal inv_d = RN(1/d): one FMA
 // correction of x*inv_d gives the correctly rounded quotient (Markstein's
 // theorem), i.e. exactly what IEEE division returns, for a product and two FMAs
 // instead of a full division sequence.
-SSPT_DH inline double div_rcp(double x, double d, double inv_d)
+template <typename R> SSPT_DH inline R div_rcp(R x, R d, R inv_d)
 {
-    const double q = x * inv_d;
-    const double r = std::fma(-q, d, x);
+    const R q = x * inv_d;
+    const R r = std::fma(-q, d, x);
     return std::fma(r, inv_d, q);
 }
 
 // x^ex of the power law.  ex = 0 (n_exp = 1) is 1 exactly, as pow returns;
 // small integer exponents (n_exp = 2..9 -> ex = -1..-8) use repeated
 // products and one reciprocal (within 2 ulp of pow); other exponents use pow.
-SSPT_DH inline double pow_ex(double x, double ex)
+template <typename R> SSPT_DH inline R pow_ex(R x, R ex)
 {
-    if (ex == 0.0) return 1.0;
-    if (ex < 0.0 && ex >= -8.0 && ex == (double)(int)ex) {
+    if (ex == R(0)) return R(1);
+    if (ex < R(0) && ex >= R(-8) && ex == (R)(int)ex) {
         const int m = -(int)ex;
-        double p = x;
+        R p = x;
         for (int i = 1; i < m; ++i) p = p * x;
-        return 1.0 / p;
+        return R(1) / p;
     }
     return std::pow(x, ex);
 }
 
 // power_law! (SphericalStokes_diapir.jl:259-265) with the optional log-space
 // relaxation (CartesianStokes_diapir.jl:210-221) and cutoffs.
-SSPT_DH inline double power_law_pt(double tii, double ini, double ePL_old, double *ePL_new, const Args &a,
-                                   int flags)
+template <typename R>
+SSPT_DH inline R power_law_pt(R tii, R ini, R ePL_old, R *ePL_new, const Args<R> &a, int flags)
 {
-    double pl = ini * pow_ex(tii / a.tC, a.ex);
-    if (flags & F_RELAX) pl = std::exp(std::log(pl) * a.relax + std::log(ePL_old) * (1.0 - a.relax));
+    R pl = ini * pow_ex(tii / a.tC, a.ex);
+    if (flags & F_RELAX) pl = std::exp(std::log(pl) * a.relax + std::log(ePL_old) * (R(1) - a.relax));
     *ePL_new = pl;
-    double e = 2.0 / (1.0 / ini + 1.0 / pl);
+    R e = R(2) / (R(1) / ini + R(1) / pl);
     if (flags & F_CLAMP) {
         e = jmax(e, a.emin);   // eta_min = 0 / eta_max = +inf leave eta unchanged
         e = jmin(e, a.emax);
@@ -149,28 +157,29 @@ SSPT_DH inline double power_law_pt(double tii, double ini, double ePL_old, doubl
     return e;
 }
 
-struct Thread {
+template <typename R> struct Thread {
     int tx, ty, j, k;
     int jc, kc;       // j,k clamped into the box for loads
     bool own;         // owned (written) position
     // registers carried along r
-    double vr0, vr1, vt0, vt1, vp0, vp1;     // old V at planes s, s+1 (own column)
-    double e0, e1, e2;                       // old eta at s, s+1, s+2
-    double nvr, nvt, nvp, ne, ringJ, ringK;  // prefetched: V(s+2), eta(s+3), eta ring of plane s+2
-    double sm0, sm1;                         // etaSM at s, s+1
-    double cRR, cTT, cPP, cDT, crho;         // sigma/DT/rho of plane s-1 (VR face s)
-    double rt0, rt0n, rp0, rp0n;             // new tau_Rth/tau_Rph of face s (own, +1 neighbour)
-    double pG_rt, pE_rt, pG_rp, pE_rp;       // partial 4-point sums of plane s for face s+1
-    double gtp, etp;                         // 4-point G / eta averages for tau_thph(s)
+    R vr0, vr1, vt0, vt1, vp0, vp1;     // old V at planes s, s+1 (own column)
+    R e0, e1, e2;                       // old eta at s, s+1, s+2
+    R nvr, nvt, nvp, ne, ringJ, ringK;  // prefetched: V(s+2), eta(s+3), eta ring of plane s+2
+    R sm0, sm1;                         // etaSM at s, s+1
+    R cRR, cTT, cPP, cDT, crho;         // sigma/DT/rho of plane s-1 (VR face s)
+    R rt0, rt0n, rp0, rp0n;             // new tau_Rth/tau_Rph of face s (own, +1 neighbour)
+    R pG_rt, pE_rt, pG_rp, pE_rp;       // partial 4-point sums of plane s for face s+1
+    R gtp, etp;                         // 4-point G / eta averages for tau_thph(s)
     // values of the current step
-    double oP, oRR, oTT, oPP, oRTold, oRPold, oTPold, orho, oini, oPL;
-    double nRR, nTT, nPP, nSRR, nSTT, nSPP, nDT;
-    double nrt, nrp, ntp;
-    double pG_rt2, pE_rt2, pG_rp2, pE_rp2, gtp2, etp2;
+    R oP, oRR, oTT, oPP, oRTold, oRPold, oTPold, orho, oini, oPL;
+    R ogP, ofR, ofT, ofP;               // forcing of a correction sweep
+    R nRR, nTT, nPP, nSRR, nSTT, nSPP, nDT;
+    R nrt, nrp, ntp;
+    R pG_rt2, pE_rt2, pG_rp2, pE_rp2, gtp2, etp2;
     unsigned long long mdv[3], mv[3];
 };
 
-SSPT_DH inline void thread_init(Thread &T, const Args &a, int tx, int ty, int kb, int jb)
+template <typename R> SSPT_DH inline void thread_init(Thread<R> &T, const Args<R> &a, int tx, int ty, int kb, int jb)
 {
     T.tx = tx;
     T.ty = SSPT_UNIFORM(ty);
@@ -180,17 +189,17 @@ SSPT_DH inline void thread_init(Thread &T, const Args &a, int tx, int ty, int kb
     T.jc = clampi(T.j, 0, a.b.nt);
     T.own = tx >= 1 && tx <= TK - 2 && ty >= 1 && ty <= TJB - 2 && T.j < a.b.nt && T.k < a.b.np;
     for (int c = 0; c < 3; ++c) T.mdv[c] = T.mv[c] = 0ull;
-    T.vr0 = T.vr1 = T.vt0 = T.vt1 = T.vp0 = T.vp1 = 0.0;
-    T.e0 = T.e1 = T.e2 = 1.0;
-    T.sm0 = T.sm1 = 1.0;
-    T.cRR = T.cTT = T.cPP = T.cDT = T.crho = 0.0;
-    T.rt0 = T.rt0n = T.rp0 = T.rp0n = 0.0;
-    T.pG_rt = T.pE_rt = T.pG_rp = T.pE_rp = T.gtp = T.etp = 1.0;
-    T.nrt = T.nrp = T.ntp = 0.0;
+    T.vr0 = T.vr1 = T.vt0 = T.vt1 = T.vp0 = T.vp1 = R(0);
+    T.e0 = T.e1 = T.e2 = R(1);
+    T.sm0 = T.sm1 = R(1);
+    T.cRR = T.cTT = T.cPP = T.cDT = T.crho = R(0);
+    T.rt0 = T.rt0n = T.rp0 = T.rp0n = R(0);
+    T.pG_rt = T.pE_rt = T.pG_rp = T.pE_rp = T.gtp = T.etp = R(1);
+    T.nrt = T.nrp = T.ntp = R(0);
 }
 
 // Row/column ring of the eta plane p for the star of the block's edge threads.
-SSPT_DH inline void load_ring(Thread &T, const Args &a, int p)
+template <typename R> SSPT_DH inline void load_ring(Thread<R> &T, const Args<R> &a, int p)
 {
     const BoxG &b = a.b;
     const int ie = clampi(p, 0, b.nr - 1);
@@ -201,41 +210,48 @@ SSPT_DH inline void load_ring(Thread &T, const Args &a, int p)
     if (T.tx == TK - 1) T.ringK = a.eta[b.off(ie, jcc, clampi(T.k + 1, 0, b.np - 1))];
 }
 
-SSPT_DH inline long long own_centre(const Thread &T, const BoxG &b, int s)
+template <typename R> SSPT_DH inline long long own_centre(const Thread<R> &T, const BoxG &b, int s)
 {
     return b.off(clampi(s, 0, b.nr - 1), clampi(T.jc, 0, b.nt - 1), clampi(T.kc, 0, b.np - 1));
 }
 
 // Own old values of step s.  They are issued one step ahead (right after
 // their previous use), so their latency hides behind a whole step.
-SSPT_DH inline void load_centre_old(Thread &T, const Args &a, int s)
+template <typename R, int FL> SSPT_DH inline void load_centre_old(Thread<R> &T, const Args<R> &a, int s)
 {
     const long long oc = own_centre(T, a.b, s);
     T.oP = a.P[oc];
     T.oRR = a.tRR[oc];
     T.oTT = a.tTT[oc];
     T.oPP = a.tPP[oc];
-    T.orho = a.rho[oc];
+    if (FL & F_CORR) {
+        T.ogP = a.gP[oc];
+        T.ofR = a.fVR[oc];
+        T.ofT = a.fVT[oc];
+        T.ofP = a.fVP[oc];
+    } else {
+        T.orho = a.rho[oc];
+    }
     T.oTPold = a.tTP[a.b.off(clampi(s, 0, a.b.nr - 1), T.jc, T.kc)];
 }
-SSPT_DH inline void load_face_old(Thread &T, const Args &a, int s)
+template <typename R> SSPT_DH inline void load_face_old(Thread<R> &T, const Args<R> &a, int s)
 {
     const long long oe = a.b.off(clampi(s + 1, 0, a.b.nr), T.jc, T.kc);
     T.oRTold = a.tRT[oe];
     T.oRPold = a.tRP[oe];
 }
-template <int FL> SSPT_DH inline void load_visc_old(Thread &T, const Args &a, int s)
+template <typename R, int FL> SSPT_DH inline void load_visc_old(Thread<R> &T, const Args<R> &a, int s)
 {
-    if (T.own) {
+    if (!(FL & F_CORR) && T.own) {
         const long long oc = own_centre(T, a.b, s);
         T.oini = a.ini[oc];
-        T.oPL = (FL & F_RELAX) ? a.ePL[oc] : 0.0;
+        T.oPL = (FL & F_RELAX) ? a.ePL[oc] : R(0);
     }
 }
 
 // Loads before the march (start = first step): the V/eta window, the
 // prefetch registers and the own old values of step `start`.
-template <int FL> SSPT_DH inline void thread_prologue(Thread &T, const Args &a, int start)
+template <typename R, int FL> SSPT_DH inline void thread_prologue(Thread<R> &T, const Args<R> &a, int start)
 {
     const BoxG &b = a.b;
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
@@ -248,17 +264,17 @@ template <int FL> SSPT_DH inline void thread_prologue(Thread &T, const Args &a, 
     T.nvr = a.VR[b.off(clampi(start + 1, 0, b.nr), T.jc, T.kc)];
     T.nvt = a.VT[b.off(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc)];
     T.nvp = a.VP[b.off(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc)];
-    T.ringJ = T.ringK = 0.0;
+    T.ringJ = T.ringK = R(0);
     load_ring(T, a, start + 1);
-    load_centre_old(T, a, start);
+    load_centre_old<R, FL>(T, a, start);
     load_face_old(T, a, start);
-    load_visc_old<FL>(T, a, start);
+    load_visc_old<R, FL>(T, a, start);
 }
 
 // ---- phase 1: stage old V and eta of the coming plane in LDS (values
 // prefetched during the previous step) and prefetch the plane after it
-template <int FL>
-SSPT_DH inline void phase1(Thread &T, Lds &L, const Args &a, int s)
+template <typename R, int FL>
+SSPT_DH inline void phase1(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s)
 {
     const BoxG &b = a.b;
     const int nr = b.nr;
@@ -293,19 +309,19 @@ SSPT_DH inline void phase1(Thread &T, Lds &L, const Args &a, int s)
 // (fmax) equals Julia's max for the positive, finite viscosities of a valid
 // state; it does not propagate a NaN viscosity into etaSM (a NaN still
 // reaches V through eta itself, so the NaN exit test still fires).
-SSPT_DH inline double star(const Lds &L, int ty, int tx, double em, double ep)
+template <typename R> SSPT_DH inline R star(const Lds<R> &L, int ty, int tx, R em, R ep)
 {
-    const double c = L.ETA[ty + 1][tx + 1];
+    const R c = L.ETA[ty + 1][tx + 1];
     return std::fmax(std::fmax(std::fmax(std::fmax(em, ep), c), std::fmax(L.ETA[ty][tx + 1], L.ETA[ty + 2][tx + 1])),
                      std::fmax(L.ETA[ty + 1][tx], L.ETA[ty + 1][tx + 2]));
 }
 
 // ---- phase 2: etaSM(s+1); C(s) = P, tau_nn, DT at centres; tau_thph(s); VR face s
-template <int FL>
-SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
+template <typename R, int FL>
+SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, bool wr)
 {
     const BoxG &b = a.b;
-    const TabsG &t = a.t;
+    const TabsG<R> &t = a.t;
     const int nr = b.nr, nt = b.nt, np = b.np;
     const int p = s + 1;
     // etaSM of plane p after copy_BC!: star at the clamped position
@@ -316,7 +332,7 @@ SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
         } else {
             const int dty = cj - T.j, dtx = ck - T.k;
             const int ty = clampi(T.ty + dty, 0, TJB - 1), tx = clampi(T.tx + dtx, 0, TK - 1);
-            const double em = a.eta[b.off(p - 1, cj, ck)], ep = a.eta[b.off(p + 1, cj, ck)];
+            const R em = a.eta[b.off(p - 1, cj, ck)], ep = a.eta[b.off(p + 1, cj, ck)];
             T.sm1 = star(L, ty, tx, em, ep);
         }
     } else if (p == nr - 1) {
@@ -328,26 +344,26 @@ SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
     const bool centre = T.j >= 0 && T.j < nt && T.k >= 0 && T.k < np;
     // C(s): compute_P! centre part
     if (s >= 0 && s <= nr - 1 && centre && T.tx < TK - 1 && T.ty < TJB - 1) {
-        const double R = t.tr[s * NTR + TR_Rc], iR = t.tr[s * NTR + TR_invRc];
-        const double dVRr = (T.vr1 - T.vr0) * a._dr;
-        const double dVTt = (iR * (L.VT[T.ty + 1][T.tx] - T.vt0)) * a._dth;
-        const double dVPp = (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c] * (L.VP[T.ty][T.tx + 1] - T.vp0)) * a._dph;
-        const double avVR = (T.vr0 + T.vr1) * 0.5;
-        const double avVT = (T.vt0 + L.VT[T.ty + 1][T.tx]) * 0.5;
-        const double ct = t.tt[T.j * NTT + TT_cotc];
-        const double vr_R = div_rcp(avVR, R, iR), vt_R = div_rcp(avVT, R, iR);
-        const double divV = (((dVRr + dVTt) + dVPp) + div_rcp(2.0 * avVR, R, iR)) + vt_R * ct;
-        const double third = 1.0 / 3.0;
-        const double Drr = dVRr - third * divV;
-        const double Dtt = (dVTt + vr_R) - third * divV;
-        const double Dpp = ((dVPp + vr_R) + vt_R * ct) - third * divV;
-        const double KB = a.kb * T.sm0, GS = a.gs * T.sm0;
-        const double den = 1.0 + GS / T.e0;
-        const double iden = 1.0 / den;
-        const double nP = T.oP - KB * divV;
-        T.nRR = div_rcp(T.oRR + (GS * 2.0) * Drr, den, iden);
-        T.nTT = div_rcp(T.oTT + (GS * 2.0) * Dtt, den, iden);
-        T.nPP = div_rcp(T.oPP + (GS * 2.0) * Dpp, den, iden);
+        const R Rv = t.tr[s * NTR + TR_Rc], iR = t.tr[s * NTR + TR_invRc];
+        const R dVRr = (T.vr1 - T.vr0) * a._dr;
+        const R dVTt = (iR * (L.VT[T.ty + 1][T.tx] - T.vt0)) * a._dth;
+        const R dVPp = (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c] * (L.VP[T.ty][T.tx + 1] - T.vp0)) * a._dph;
+        const R avVR = (T.vr0 + T.vr1) * R(0.5);
+        const R avVT = (T.vt0 + L.VT[T.ty + 1][T.tx]) * R(0.5);
+        const R ct = t.tt[T.j * NTT + TT_cotc];
+        const R vr_R = div_rcp(avVR, Rv, iR), vt_R = div_rcp(avVT, Rv, iR);
+        const R divV = (((dVRr + dVTt) + dVPp) + div_rcp(R(2) * avVR, Rv, iR)) + vt_R * ct;
+        const R third = R(1) / R(3);
+        const R Drr = dVRr - third * divV;
+        const R Dtt = (dVTt + vr_R) - third * divV;
+        const R Dpp = ((dVPp + vr_R) + vt_R * ct) - third * divV;
+        const R KB = a.kb * T.sm0, GS = a.gs * T.sm0;
+        const R den = R(1) + GS / T.e0;
+        const R iden = R(1) / den;
+        const R nP = (FL & F_CORR) ? T.oP - KB * (divV + T.ogP) : T.oP - KB * divV;
+        T.nRR = div_rcp(T.oRR + (GS * R(2)) * Drr, den, iden);
+        T.nTT = div_rcp(T.oTT + (GS * R(2)) * Dtt, den, iden);
+        T.nPP = div_rcp(T.oPP + (GS * R(2)) * Dpp, den, iden);
         T.nDT = a.vpdt2 / (KB + div_rcp(GS, den, iden));
         T.nSRR = -nP + T.nRR;
         T.nSTT = -nP + T.nTT;
@@ -358,108 +374,113 @@ SSPT_DH inline void phase2(Thread &T, Lds &L, const Args &a, int s, bool wr)
             a.oRR[o] = T.nRR;
             a.oTT[o] = T.nTT;
             a.oPP[o] = T.nPP;
-            if (FL & F_LAST) {
+            if ((FL & F_LAST) && !(FL & F_CORR)) {
                 a.DT[o] = T.nDT;
                 a.SM[o] = T.sm0;
             }
         }
     } else {
-        T.nRR = T.nTT = T.nPP = 0.0;
-        T.nSRR = T.nSTT = T.nSPP = 0.0;
-        T.nDT = 0.0;
+        T.nRR = T.nTT = T.nPP = R(0);
+        T.nSRR = T.nSTT = T.nSPP = R(0);
+        T.nDT = R(0);
     }
     L.STT[T.ty][T.tx] = T.nSTT;
     L.SPP[T.ty][T.tx] = T.nSPP;
     L.DT[T.ty][T.tx] = T.nDT;
 
     // tau_thph(s): compute_P! edge part, plane s, th-face j, phi-face k
-    T.ntp = 0.0;
+    T.ntp = R(0);
     if (s >= 1 && s <= nr - 2 && T.tx >= 1 && T.ty >= 1 && T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 1) {
-        const double dVP = T.vp0 - L.VP[T.ty - 1][T.tx];
-        const double dVT = T.vt0 - L.VT[T.ty][T.tx - 1];
-        const double avVP = (L.VP[T.ty - 1][T.tx] + T.vp0) * 0.5;
-        const double D = ((t.tr[s * NTR + TR_invRc4] * dVP) * a._dth + (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c4] * dVT) * a._dph) -
+        const R dVP = T.vp0 - L.VP[T.ty - 1][T.tx];
+        const R dVT = T.vt0 - L.VT[T.ty][T.tx - 1];
+        const R avVP = (L.VP[T.ty - 1][T.tx] + T.vp0) * R(0.5);
+        const R D = ((t.tr[s * NTR + TR_invRc4] * dVP) * a._dth + (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c4] * dVT) * a._dph) -
                          div_rcp(avVP, t.tr[s * NTR + TR_Rc4], t.tr[s * NTR + TR_invRc4]) * t.tt[T.j * NTT + TT_cotf4];
-        T.ntp = (T.oTPold + T.gtp * D) / (1.0 + T.gtp / T.etp);
+        T.ntp = (T.oTPold + T.gtp * D) / (R(1) + T.gtp / T.etp);
         if (wr && T.own) a.oTP[b.off(s, T.j, T.k)] = T.ntp;
     }
     L.TP[T.ty][T.tx] = T.ntp;
 
     // VR at r-face s (compute_V!): needs C(s-1), C(s), tau_Rth/tau_Rph of face s
     if (wr && T.own && s >= 1 && s <= nr - 1 && T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 2) {
-        const double Rx = t.tr[s * NTR + TR_Rf], iRx = t.tr[s * NTR + TR_invRf];
-        const double sRR = (T.cRR + T.nSRR) * 0.5;
-        const double sTT = (T.cTT + T.nSTT) * 0.5;
-        const double sPP = (T.cPP + T.nSPP) * 0.5;
-        const double aRT = (T.rt0 + T.rt0n) * 0.5;
-        const double rg = (T.crho * a.g + T.orho * a.g) * 0.5;
-        const double dV = ((((((T.nSRR - T.cRR) * a._dr + (iRx * (T.rt0n - T.rt0)) * a._dth) +
-                              (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_f] * (T.rp0n - T.rp0)) * a._dph) +
-                             div_rcp(2.0 * sRR, Rx, iRx)) -
-                            div_rcp(sTT + sPP, Rx, iRx)) +
-                           div_rcp(aRT, Rx, iRx) * t.tt[T.j * NTT + TT_cotc2]) -
-                          rg;
-        const double dt = (T.cDT + T.nDT) * 0.5;
-        const double v = T.vr0 + dV * dt;
+        const R Rx = t.tr[s * NTR + TR_Rf], iRx = t.tr[s * NTR + TR_invRf];
+        const R sRR = (T.cRR + T.nSRR) * R(0.5);
+        const R sTT = (T.cTT + T.nSTT) * R(0.5);
+        const R sPP = (T.cPP + T.nSPP) * R(0.5);
+        const R aRT = (T.rt0 + T.rt0n) * R(0.5);
+        const R dsig = (((((T.nSRR - T.cRR) * a._dr + (iRx * (T.rt0n - T.rt0)) * a._dth) +
+                            (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_f] * (T.rp0n - T.rp0)) * a._dph) +
+                           div_rcp(R(2) * sRR, Rx, iRx)) -
+                          div_rcp(sTT + sPP, Rx, iRx)) +
+                         div_rcp(aRT, Rx, iRx) * t.tt[T.j * NTT + TT_cotc2];
+        R dV;
+        if (FL & F_CORR) {
+            dV = dsig + T.ofR;
+        } else {
+            const R rg = (T.crho * a.g + T.orho * a.g) * R(0.5);
+            dV = dsig - rg;
+        }
+        const R dt = (T.cDT + T.nDT) * R(0.5);
+        const R v = T.vr0 + dV * dt;
         const long long o = b.off(s, T.j, T.k);
         a.oVR[o] = v;
         if (FL & F_CHECK) {
-            const double d = v - T.vr0;
-            a.VRe[o] = d;
+            const R d = v - T.vr0;
+            if (!(FL & F_CORR)) a.VRe[o] = d;
             const unsigned long long ud = absbits(d), uv = absbits(v);
             T.mdv[0] = ud > T.mdv[0] ? ud : T.mdv[0];
             T.mv[0] = uv > T.mv[0] ? uv : T.mv[0];
-        } else if (FL & F_LAST) {
+        } else if ((FL & F_LAST) && !(FL & F_CORR)) {
             a.VRe[o] = T.vr0;
         }
     }
     T.crho = T.orho;
-    load_centre_old(T, a, s + 1);
+    load_centre_old<R, FL>(T, a, s + 1);
 }
 
 // ---- phase 3: tau_Rth / tau_Rph on r-face s+1; 4-point sums of plane s+1
-template <int FL>
-SSPT_DH inline void phase3(Thread &T, Lds &L, const Args &a, int s, bool wr)
+template <typename R, int FL>
+SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, bool wr)
 {
     const BoxG &b = a.b;
-    const TabsG &t = a.t;
+    const TabsG<R> &t = a.t;
     const int nr = b.nr, nt = b.nt, np = b.np;
     const int f = s + 1;
-    const double gs = a.gs;
+    const R gs = a.gs;
     const bool inner = T.tx >= 1 && T.ty >= 1;
     // sums over plane s+1 (etaSM(s+1) in L.SM, eta(s+1) in L.ETA)
     if (inner) {
-        const double smJ = L.SM[T.ty - 1][T.tx], smK = L.SM[T.ty][T.tx - 1], smJK = L.SM[T.ty - 1][T.tx - 1];
-        const double eJ = L.ETA[T.ty][T.tx + 1], eK = L.ETA[T.ty + 1][T.tx], eJK = L.ETA[T.ty][T.tx];
+        const R smJ = L.SM[T.ty - 1][T.tx], smK = L.SM[T.ty][T.tx - 1], smJK = L.SM[T.ty - 1][T.tx - 1];
+        const R eJ = L.ETA[T.ty][T.tx + 1], eK = L.ETA[T.ty + 1][T.tx], eJK = L.ETA[T.ty][T.tx];
         T.pG_rt2 = gs * smJ + gs * T.sm1;
         T.pE_rt2 = eJ + T.e1;
         T.pG_rp2 = gs * smK + gs * T.sm1;
         T.pE_rp2 = eK + T.e1;
-        T.gtp2 = (((gs * smJK + gs * smJ) + gs * smK) + gs * T.sm1) * 0.25;
-        T.etp2 = (((eJK + eJ) + eK) + T.e1) * 0.25;
+        T.gtp2 = (((gs * smJK + gs * smJ) + gs * smK) + gs * T.sm1) * R(0.25);
+        T.etp2 = (((eJK + eJ) + eK) + T.e1) * R(0.25);
         if (s == 0) { // etaSM(0,:,:) = etaSM(1,:,:): plane-0 sums of gs*etaSM are those of plane 1
             T.pG_rt = T.pG_rt2;
             T.pG_rp = T.pG_rp2;
         }
     }
-    T.nrt = 0.0;
-    T.nrp = 0.0;
+    T.nrt = R(0);
+    T.nrp = R(0);
     if (f >= 1 && f <= nr - 1 && inner) {
-        const double Re = t.tr[f * NTR + TR_Re], iRe = t.tr[f * NTR + TR_invRe];
+        const R Re = t.tr[f * NTR + TR_Re], iRe = t.tr[f * NTR + TR_invRe];
         if (T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 2) {
-            const double G = ((T.pG_rt + gs * L.SM[T.ty - 1][T.tx]) + gs * T.sm1) * 0.25;
-            const double E = ((T.pE_rt + L.ETA[T.ty][T.tx + 1]) + T.e1) * 0.25;
-            const double D = ((T.vt1 - T.vt0) * a._dr + (iRe * (T.vr1 - L.VR[T.ty - 1][T.tx])) * a._dth) -
-                             div_rcp((T.vt0 + T.vt1) * 0.5, Re, iRe);
-            T.nrt = (T.oRTold + G * D) / (1.0 + G / E);
+            const R G = ((T.pG_rt + gs * L.SM[T.ty - 1][T.tx]) + gs * T.sm1) * R(0.25);
+            const R E = ((T.pE_rt + L.ETA[T.ty][T.tx + 1]) + T.e1) * R(0.25);
+            const R D = ((T.vt1 - T.vt0) * a._dr + (iRe * (T.vr1 - L.VR[T.ty - 1][T.tx])) * a._dth) -
+                             div_rcp((T.vt0 + T.vt1) * R(0.5), Re, iRe);
+            T.nrt = (T.oRTold + G * D) / (R(1) + G / E);
             if (wr && T.own) a.oRT[b.off(f, T.j, T.k)] = T.nrt;
         }
         if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
-            const double G = ((T.pG_rp + gs * L.SM[T.ty][T.tx - 1]) + gs * T.sm1) * 0.25;
-            const double E = ((T.pE_rp + L.ETA[T.ty + 1][T.tx]) + T.e1) * 0.25;
-            const double D = ((T.vp1 - T.vp0) * a._dr + (t.t2[(f * (nt + 1) + T.j) * NT2 + T2_e] * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
-                             div_rcp((T.vp0 + T.vp1) * 0.5, Re, iRe);
-            T.nrp = (T.oRPold + G * D) / (1.0 + G / E);
+            const R G = ((T.pG_rp + gs * L.SM[T.ty][T.tx - 1]) + gs * T.sm1) * R(0.25);
+            const R E = ((T.pE_rp + L.ETA[T.ty + 1][T.tx]) + T.e1) * R(0.25);
+            const R D = ((T.vp1 - T.vp0) * a._dr + (t.t2[(f * (nt + 1) + T.j) * NT2 + T2_e] * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
+                             div_rcp((T.vp0 + T.vp1) * R(0.5), Re, iRe);
+            T.nrp = (T.oRPold + G * D) / (R(1) + G / E);
             if (wr && T.own) a.oRP[b.off(f, T.j, T.k)] = T.nrp;
         }
     }
@@ -469,71 +490,73 @@ SSPT_DH inline void phase3(Thread &T, Lds &L, const Args &a, int s, bool wr)
 }
 
 // ---- phase 4: Vth(s), Vph(s), tII(s) -> eta(s); shift carries
-template <int FL>
-SSPT_DH inline void phase4(Thread &T, Lds &L, const Args &a, int s, bool wr)
+template <typename R, int FL>
+SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, bool wr)
 {
     const BoxG &b = a.b;
-    const TabsG &t = a.t;
+    const TabsG<R> &t = a.t;
     const int nr = b.nr, nt = b.nt, np = b.np;
     if (wr && T.own && s >= 1 && s <= nr - 2) {
-        const double Ry = t.tr[s * NTR + TR_Rc2], iRy = t.tr[s * NTR + TR_invRc2];
+        const R Ry = t.tr[s * NTR + TR_Rc2], iRy = t.tr[s * NTR + TR_invRc2];
         // Vth at (s, j, k)
         if (T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 2) {
-            const double sTTm = L.STT[T.ty - 1][T.tx], sPPm = L.SPP[T.ty - 1][T.tx];
-            const double sTT = (sTTm + T.nSTT) * 0.5, sPP = (sPPm + T.nSPP) * 0.5;
-            const double aRT = (T.rt0 + T.nrt) * 0.5;
-            const double tpK = L.TP[T.ty][T.tx + 1];
-            const double dV = ((((T.nrt - T.rt0) * a._dr + (iRy * (T.nSTT - sTTm)) * a._dth) +
+            const R sTTm = L.STT[T.ty - 1][T.tx], sPPm = L.SPP[T.ty - 1][T.tx];
+            const R sTT = (sTTm + T.nSTT) * R(0.5), sPP = (sPPm + T.nSPP) * R(0.5);
+            const R aRT = (T.rt0 + T.nrt) * R(0.5);
+            const R tpK = L.TP[T.ty][T.tx + 1];
+            R dV = ((((T.nrt - T.rt0) * a._dr + (iRy * (T.nSTT - sTTm)) * a._dth) +
                                 (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_y] * (tpK - T.ntp)) * a._dph) +
-                               div_rcp(3.0 * aRT, Ry, iRy)) +
+                               div_rcp(R(3) * aRT, Ry, iRy)) +
                               div_rcp(sTT - sPP, Ry, iRy) * t.tt[T.j * NTT + TT_cotf2];
-            const double dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * 0.5;
-            const double v = T.vt0 + dV * dt;
+            if (FL & F_CORR) dV = dV + T.ofT;
+            const R dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * R(0.5);
+            const R v = T.vt0 + dV * dt;
             const long long o = b.off(s, T.j, T.k);
             a.oVT[o] = v;
             if (FL & F_CHECK) {
-                const double d = v - T.vt0;
-                a.VTe[o] = d;
+                const R d = v - T.vt0;
+                if (!(FL & F_CORR)) a.VTe[o] = d;
                 const unsigned long long ud = absbits(d), uv = absbits(v);
                 T.mdv[1] = ud > T.mdv[1] ? ud : T.mdv[1];
                 T.mv[1] = uv > T.mv[1] ? uv : T.mv[1];
-            } else if (FL & F_LAST) {
+            } else if ((FL & F_LAST) && !(FL & F_CORR)) {
                 a.VTe[o] = T.vt0;
             }
         }
         // Vph at (s, j, k)
         if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
-            const double tpJ = L.TP[T.ty + 1][T.tx];
-            const double sPPm = L.SPP[T.ty][T.tx - 1];
-            const double aRP = (T.rp0 + T.nrp) * 0.5;
-            const double aTP = (T.ntp + tpJ) * 0.5;
-            const double dV = ((((T.nrp - T.rp0) * a._dr + (iRy * (tpJ - T.ntp)) * a._dth) +
+            const R tpJ = L.TP[T.ty + 1][T.tx];
+            const R sPPm = L.SPP[T.ty][T.tx - 1];
+            const R aRP = (T.rp0 + T.nrp) * R(0.5);
+            const R aTP = (T.ntp + tpJ) * R(0.5);
+            R dV = ((((T.nrp - T.rp0) * a._dr + (iRy * (tpJ - T.ntp)) * a._dth) +
                                 (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_z] * (T.nSPP - sPPm)) * a._dph) +
-                               div_rcp(3.0 * aRP, Ry, iRy)) +
-                              div_rcp(2.0 * aTP, Ry, iRy) * t.tt[T.j * NTT + TT_cotc2];
-            const double dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * 0.5;
-            const double v = T.vp0 + dV * dt;
+                               div_rcp(R(3) * aRP, Ry, iRy)) +
+                              div_rcp(R(2) * aTP, Ry, iRy) * t.tt[T.j * NTT + TT_cotc2];
+            if (FL & F_CORR) dV = dV + T.ofP;
+            const R dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * R(0.5);
+            const R v = T.vp0 + dV * dt;
             const long long o = b.off(s, T.j, T.k);
             a.oVP[o] = v;
             if (FL & F_CHECK) {
-                const double d = v - T.vp0;
-                a.VPe[o] = d;
+                const R d = v - T.vp0;
+                if (!(FL & F_CORR)) a.VPe[o] = d;
                 const unsigned long long ud = absbits(d), uv = absbits(v);
                 T.mdv[2] = ud > T.mdv[2] ? ud : T.mdv[2];
                 T.mv[2] = uv > T.mv[2] ? uv : T.mv[2];
-            } else if (FL & F_LAST) {
+            } else if ((FL & F_LAST) && !(FL & F_CORR)) {
                 a.VPe[o] = T.vp0;
             }
         }
-        // tII and the new viscosity of interior cells
-        if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 2) {
-            const double x = (((T.rt0 + T.rt0n) + T.nrt) + L.RT[T.ty + 1][T.tx]) * 0.25;
-            const double y = (((T.rp0 + T.rp0n) + T.nrp) + L.RP[T.ty][T.tx + 1]) * 0.25;
-            const double z = (((T.ntp + L.TP[T.ty][T.tx + 1]) + L.TP[T.ty + 1][T.tx]) + L.TP[T.ty + 1][T.tx + 1]) * 0.25;
-            const double tii =
-                std::sqrt((((1.0 / 2.0) * ((T.nRR * T.nRR + T.nTT * T.nTT) + T.nPP * T.nPP) + x * x) + y * y) + z * z);
-            double pl;
-            const double e = power_law_pt(tii, T.oini, T.oPL, &pl, a, FL);
+        // tII and the new viscosity of interior cells (eta is frozen in a correction sweep)
+        if (!(FL & F_CORR) && T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 2) {
+            const R x = (((T.rt0 + T.rt0n) + T.nrt) + L.RT[T.ty + 1][T.tx]) * R(0.25);
+            const R y = (((T.rp0 + T.rp0n) + T.nrp) + L.RP[T.ty][T.tx + 1]) * R(0.25);
+            const R z = (((T.ntp + L.TP[T.ty][T.tx + 1]) + L.TP[T.ty + 1][T.tx]) + L.TP[T.ty + 1][T.tx + 1]) * R(0.25);
+            const R tii =
+                std::sqrt((((R(1) / R(2)) * ((T.nRR * T.nRR + T.nTT * T.nTT) + T.nPP * T.nPP) + x * x) + y * y) + z * z);
+            R pl;
+            const R e = power_law_pt(tii, T.oini, T.oPL, &pl, a, FL);
             const long long o = b.off(s, T.j, T.k);
             a.oEta[o] = e;
             if (FL & F_RELAX) a.oPL[o] = pl;
@@ -558,7 +581,7 @@ SSPT_DH inline void phase4(Thread &T, Lds &L, const Args &a, int s, bool wr)
     T.pE_rp = T.pE_rp2;
     T.gtp = T.gtp2;
     T.etp = T.etp2;
-    load_visc_old<FL>(T, a, s + 1);
+    load_visc_old<R, FL>(T, a, s + 1);
 }
 
 // ---------------------------------------------------------------- boundary pass
@@ -579,7 +602,7 @@ SSPT_DH inline bool ghost_dir(const BoxG &b, int comp, int dir, int idx, int *in
 }
 
 // affine BC map: value = a*v + c  (a in {-1,0,1})
-SSPT_DH inline double bc_apply(const Args &A, int comp, int dir, int side, int j, double v)
+template <typename R> SSPT_DH inline R bc_apply(const Args<R> &A, int comp, int dir, int side, int j, R v)
 {
     const BoxG &b = A.b;
     const int mode = A.bc_mode; // 0 shear, 1 free slip, 2 pure shear
@@ -587,31 +610,31 @@ SSPT_DH inline double bc_apply(const Args &A, int comp, int dir, int side, int j
     if (dir == 1) {
         if (comp == 1) {
             if (mode == 2) {
-                const double th = side ? A.t.th[b.nt - 1] : A.t.th[0];
-                const double vb = -(((th - PI_ / 2.0) * A.r) * A.sh);
-                return 2.0 * vb - v;
+                const R th = side ? A.t.th[b.nt - 1] : A.t.th[0];
+                const R vb = -(((th - R(PI_ / 2.0)) * A.r) * A.sh);
+                return R(2) * vb - v;
             }
             return -v;
         }
         if (mode == 0) {
-            if (comp == 0) return 0.0;
-            const double th = side ? A.t.th[b.nt - 1] : A.t.th[0];
-            return ((th - PI_ / 2.0) * A.r) * A.sh;
+            if (comp == 0) return R(0);
+            const R th = side ? A.t.th[b.nt - 1] : A.t.th[0];
+            return ((th - R(PI_ / 2.0)) * A.r) * A.sh;
         }
         return v;
     }
     // dir == 2
     if (comp == 2) {
-        if (mode == 0) return ((2.0 * (A.t.th[j] - PI_ / 2.0)) * A.r) * A.sh - v;
+        if (mode == 0) return ((R(2) * (A.t.th[j] - R(PI_ / 2.0))) * A.r) * A.sh - v;
         if (mode == 1) return -v;
-        const double ph = side ? A.t.ph[b.np - 1] : A.t.ph[0];
-        return 2.0 * ((ph * A.r) * A.sh) - v;
+        const R ph = side ? A.t.ph[b.np - 1] : A.t.ph[0];
+        return R(2) * ((ph * A.r) * A.sh) - v;
     }
-    if (mode == 0) return 0.0;
+    if (mode == 0) return R(0);
     return v;
 }
 
-SSPT_DH inline bool bc_is_const(const Args &A, int comp, int dir)
+template <typename R> SSPT_DH inline bool bc_is_const(const Args<R> &A, int comp, int dir)
 {
     if (A.bc_mode != 0) return false;
     if (dir == 1) return comp != 1;
@@ -619,7 +642,7 @@ SSPT_DH inline bool bc_is_const(const Args &A, int comp, int dir)
     return false;
 }
 
-SSPT_DH inline double ghost_value(const Args &A, const double *V, int comp, int i, int j, int k)
+template <typename R> SSPT_DH inline R ghost_value(const Args<R> &A, const R *V, int comp, int i, int j, int k)
 {
     const BoxG &b = A.b;
     int ii[3] = {i, j, k};
@@ -636,7 +659,7 @@ SSPT_DH inline double ghost_value(const Args &A, const double *V, int comp, int 
             if (bc_is_const(A, comp, d)) { konst = true; break; }
         }
     }
-    double v = konst ? 0.0 : V[b.off(ii[0], ii[1], ii[2])];
+    R v = konst ? R(0) : V[b.off(ii[0], ii[1], ii[2])];
     for (int q = nd - 1; q >= 0; --q) {
         // j of the position at which this BC is applied (ii[1] restored up to this level)
         v = bc_apply(A, comp, dirs[q], sides[q], j, v);
@@ -645,49 +668,49 @@ SSPT_DH inline double ghost_value(const Args &A, const double *V, int comp, int 
 }
 
 // tII of interior cell (i,j,k) from the new stresses (compute_TII!)
-SSPT_DH inline double tii_inner(const Args &a, int i, int j, int k)
+template <typename R> SSPT_DH inline R tii_inner(const Args<R> &a, int i, int j, int k)
 {
     const BoxG &b = a.b;
     const long long o = b.off(i, j, k);
-    const double A = a.oRR[o], B = a.oTT[o], C = a.oPP[o];
-    const double *RT = a.oRT, *RP = a.oRP, *TP = a.oTP;
-    const double x = (((RT[o] + RT[o + b.sj]) + RT[o + b.si]) + RT[o + b.si + b.sj]) * 0.25;
-    const double y = (((RP[o] + RP[o + 1]) + RP[o + b.si]) + RP[o + b.si + 1]) * 0.25;
-    const double z = (((TP[o] + TP[o + 1]) + TP[o + b.sj]) + TP[o + b.sj + 1]) * 0.25;
-    return std::sqrt((((1.0 / 2.0) * ((A * A + B * B) + C * C) + x * x) + y * y) + z * z);
+    const R A = a.oRR[o], B = a.oTT[o], C = a.oPP[o];
+    const R *RT = a.oRT, *RP = a.oRP, *TP = a.oTP;
+    const R x = (((RT[o] + RT[o + b.sj]) + RT[o + b.si]) + RT[o + b.si + b.sj]) * R(0.25);
+    const R y = (((RP[o] + RP[o + 1]) + RP[o + b.si]) + RP[o + b.si + 1]) * R(0.25);
+    const R z = (((TP[o] + TP[o + 1]) + TP[o + b.sj]) + TP[o + b.sj + 1]) * R(0.25);
+    return std::sqrt((((R(1) / R(2)) * ((A * A + B * B) + C * C) + x * x) + y * y) + z * z);
 }
 
-SSPT_DH inline void ghost_v_point(const Args &a, int comp, int i, int j, int k, unsigned long long *mdv,
+template <typename R> SSPT_DH inline void ghost_v_point(const Args<R> &a, int comp, int i, int j, int k, unsigned long long *mdv,
                                   unsigned long long *mv)
 {
     const BoxG &b = a.b;
-    const double *Vn = comp == 0 ? a.oVR : (comp == 1 ? a.oVT : a.oVP);
-    double *Vw = comp == 0 ? a.oVR : (comp == 1 ? a.oVT : a.oVP);
-    const double *Vo = comp == 0 ? a.VR : (comp == 1 ? a.VT : a.VP);
-    double *Ve = comp == 0 ? a.VRe : (comp == 1 ? a.VTe : a.VPe);
-    const double v = ghost_value(a, Vn, comp, i, j, k);
+    const R *Vn = comp == 0 ? a.oVR : (comp == 1 ? a.oVT : a.oVP);
+    R *Vw = comp == 0 ? a.oVR : (comp == 1 ? a.oVT : a.oVP);
+    const R *Vo = comp == 0 ? a.VR : (comp == 1 ? a.VT : a.VP);
+    R *Ve = comp == 0 ? a.VRe : (comp == 1 ? a.VTe : a.VPe);
+    const R v = ghost_value(a, Vn, comp, i, j, k);
     const long long o = b.off(i, j, k);
     Vw[o] = v;
     if (a.flags & F_CHECK) {
-        const double d = v - Vo[o];
-        Ve[o] = d;
+        const R d = v - Vo[o];
+        if (!(a.flags & F_CORR)) Ve[o] = d;
         const unsigned long long ud = absbits(d), uv = absbits(v);
         *mdv = ud > *mdv ? ud : *mdv;
         *mv = uv > *mv ? uv : *mv;
-    } else if (a.flags & F_LAST) {
+    } else if ((a.flags & F_LAST) && !(a.flags & F_CORR)) {
         Ve[o] = Vo[o];
     }
 }
 
-SSPT_DH inline void boundary_eta_point(const Args &a, int i, int j, int k)
+template <typename R> SSPT_DH inline void boundary_eta_point(const Args<R> &a, int i, int j, int k)
 {
     const BoxG &b = a.b;
     const int ci = clampi(i, 1, b.nr - 2), cj = clampi(j, 1, b.nt - 2), ck = clampi(k, 1, b.np - 2);
-    const double tii = tii_inner(a, ci, cj, ck);
+    const R tii = tii_inner(a, ci, cj, ck);
     const long long o = b.off(i, j, k);
-    double pl;
-    const double oldpl = (a.flags & F_RELAX) ? a.ePL[o] : 0.0;
-    const double e = power_law_pt(tii, a.ini[o], oldpl, &pl, a, a.flags);
+    R pl;
+    const R oldpl = (a.flags & F_RELAX) ? a.ePL[o] : R(0);
+    const R e = power_law_pt(tii, a.ini[o], oldpl, &pl, a, a.flags);
     a.oEta[o] = e;
     if (a.flags & F_RELAX) a.oPL[o] = pl;
     if (a.flags & F_LAST) a.TII[o] = tii;

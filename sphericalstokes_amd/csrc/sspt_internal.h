@@ -67,6 +67,11 @@ struct FieldSet {
     double *f[SS_NFIELDS];
 };
 
+// FP32 boxes of the defect-correction solve: two correction sets
+// (VR, Vth, Vph, P, tau x6 at 0..9 and 10..19), the frozen viscosity and the
+// forcing (divV and the momentum residual of the FP64 iterate).
+enum { SS_F32_ETA = 20, SS_F32_GP = 21, SS_F32_FVR = 22, SS_F32_FVT = 23, SS_F32_FVP = 24, SS_F32_NBOX = 25 };
+
 } // namespace sspt
 
 struct ss_ctx {
@@ -81,6 +86,11 @@ struct ss_ctx {
     sspt::Tables tab{};
     double *ftab_mem = nullptr;     // interleaved tables of the fused kernel
     const double *ftr = nullptr, *ftt = nullptr, *ft2 = nullptr;
+    float *ftab32_mem = nullptr;    // the same tables (and th, ph) rounded to FP32
+    const float *ftr32 = nullptr, *ftt32 = nullptr, *ft232 = nullptr, *th32 = nullptr, *ph32 = nullptr;
+    double *tau_scr[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // residual scratch
+    float *f32[sspt::SS_F32_NBOX] = {};
+    bool f32_ready = false;
     bool grid_set = false;
     double *staging = nullptr;      // upload/download staging (largest field)
     size_t staging_elems = 0;

@@ -22,7 +22,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import SSError, ss_params, ss_result
+from ._lib import SSError, ss_mixed_opts, ss_mixed_result, ss_params, ss_result
 from .fields import FIELD_ID, FIELDS, LAYOUT_C, LAYOUT_JULIA, canon, shape
 from .setups import Params, Setup
 
@@ -192,6 +192,38 @@ class Solver:
             self._chk(rc, "pt_solve")
         return {"ittot": res.ittot, "iter_end": res.iter_end, "err": res.err, "err_comp": tuple(res.err_comp),
                 "seconds": res.seconds, "status": status}
+
+    def residual(self, p: Params):
+        """(r, r_div, r_mom) of the current state with tau replaced by the
+        stress consistent with V and eta (the state is not modified)."""
+        sp = ss_params.from_params(p)
+        out = (C.c_double * 3)()
+        self._chk(self._L.ss_residual(self._h, C.byref(sp), out), "residual")
+        return tuple(out)
+
+    def pt_solve_residual(self, p: Params, maxiter: int, ncheck: int, tol: float):
+        """Plain FP64 fused iterations until residual <= tol (checked every ncheck)."""
+        sp = ss_params.from_params(p)
+        res = ss_result()
+        rc = self._L.ss_pt_solve_residual(self._h, C.byref(sp), int(maxiter), int(ncheck), float(tol), C.byref(res))
+        if rc not in (0, _lib.SS_NOT_CONVERGED):
+            self._chk(rc, "pt_solve_residual")
+        return {"iters": res.ittot, "iter_end": res.iter_end, "r": res.err, "r_div": res.err_comp[0],
+                "r_mom": res.err_comp[1], "seconds": res.seconds, "converged": rc == 0}
+
+    def pt_solve_mixed(self, p: Params, *, max_outer=20, max_inner=2000, nout_inner=100, tol_inner=1e-4,
+                       tol=1e-8):
+        """FP32 sweeps inside an FP64 defect-correction loop (ss_pt_solve_mixed)."""
+        sp = ss_params.from_params(p)
+        o = ss_mixed_opts(int(max_outer), int(max_inner), int(nout_inner), float(tol_inner), float(tol))
+        r = ss_mixed_result()
+        rc = self._L.ss_pt_solve_mixed(self._h, C.byref(sp), C.byref(o), C.byref(r))
+        if rc not in (0, _lib.SS_NOT_CONVERGED):
+            self._chk(rc, "pt_solve_mixed")
+        n = r.outer
+        return {"outer": n, "inner_total": r.inner_total, "converged": bool(r.converged), "r": r.r,
+                "r_hist": list(r.r_hist[:n + 1]), "inner_hist": list(r.inner_hist[:n]),
+                "err_inner_hist": list(r.err_inner_hist[:n]), "seconds": r.seconds}
 
     def pt_iterate_timed(self, p: Params, niter: int):
         """pt_iterate with HIP events around each main-kernel launch; returns

@@ -27,33 +27,33 @@ struct HostBox {
     long long sj, si;
 };
 
-template <int FL>
-void run_main_t(const Args &a, int rchunk, unsigned long long red[6])
+template <typename R, int FL>
+void run_main_t(const Args<R> &a, int rchunk, unsigned long long red[6])
 {
     const int nx = (a.b.np + OK_ - 1) / OK_;
     const int ny = (a.b.nt + OJ - 1) / OJ;
     const int nz = (a.b.nr + rchunk - 1) / rchunk;
-    std::unique_ptr<Lds> L(new Lds());
-    std::vector<Thread> th(TK * TJB);
+    std::unique_ptr<Lds<R>> L(new Lds<R>());
+    std::vector<Thread<R>> th(TK * TJB);
     for (int bz = 0; bz < nz; ++bz)
         for (int by = 0; by < ny; ++by)
             for (int bx = 0; bx < nx; ++bx) {
-                memset(L.get(), 0xff, sizeof(Lds)); // NaN garbage: unset LDS reads must not reach outputs
+                memset(L.get(), 0xff, sizeof(Lds<R>)); // NaN garbage: unset LDS reads must not reach outputs
                 const int kb = bx * OK_, jb = by * OJ;
                 const int s0 = bz * rchunk, s1 = std::min(s0 + rchunk, a.b.nr);
                 const int start = std::max(s0 - 2, -1);
                 for (int ty = 0; ty < TJB; ++ty)
                     for (int tx = 0; tx < TK; ++tx) {
-                        Thread &T = th[ty * TK + tx];
+                        Thread<R> &T = th[ty * TK + tx];
                         thread_init(T, a, tx, ty, kb, jb);
-                        thread_prologue<FL>(T, a, start);
+                        thread_prologue<R, FL>(T, a, start);
                     }
                 for (int s = start; s < s1; ++s) {
                     const bool wr = s >= s0;
-                    for (auto &T : th) phase1<FL>(T, *L, a, s);
-                    for (auto &T : th) phase2<FL>(T, *L, a, s, wr);
-                    for (auto &T : th) phase3<FL>(T, *L, a, s, wr);
-                    for (auto &T : th) phase4<FL>(T, *L, a, s, wr);
+                    for (auto &T : th) phase1<R, FL>(T, *L, a, s);
+                    for (auto &T : th) phase2<R, FL>(T, *L, a, s, wr);
+                    for (auto &T : th) phase3<R, FL>(T, *L, a, s, wr);
+                    for (auto &T : th) phase4<R, FL>(T, *L, a, s, wr);
                 }
                 for (auto &T : th)
                     for (int c = 0; c < 3; ++c) {
@@ -63,18 +63,18 @@ void run_main_t(const Args &a, int rchunk, unsigned long long red[6])
             }
 }
 
-void run_main(const Args &a, int rchunk, unsigned long long red[6])
+template <typename R> void run_main(const Args<R> &a, int rchunk, unsigned long long red[6])
 {
     switch (a.flags) {
-#define SSPT_FL(v) case v: run_main_t<v>(a, rchunk, red); break;
+#define SSPT_FL(v) case v: run_main_t<R, v>(a, rchunk, red); break;
     SSPT_FL(0) SSPT_FL(1) SSPT_FL(3) SSPT_FL(4) SSPT_FL(5) SSPT_FL(7)
-    SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15)
+    SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15) SSPT_FL(16) SSPT_FL(18)
 #undef SSPT_FL
     default: break;
     }
 }
 
-void run_boundary(const Args &a, unsigned long long red[6])
+template <typename R> void run_boundary(const Args<R> &a, unsigned long long red[6])
 {
     const int nr = a.b.nr, nt = a.b.nt, np = a.b.np;
     const int m = std::max(std::max(nr, nt), np) + 1;
@@ -95,7 +95,8 @@ void run_boundary(const Args &a, unsigned long long red[6])
                         ghost_v_point(a, c, v, u, n2 - 1, &mdv[c], &mv[c]);
                     }
                 }
-                if (fam == 0 && u < np && v < nt) {
+                if (a.flags & F_CORR) {
+                } else if (fam == 0 && u < np && v < nt) {
                     boundary_eta_point(a, 0, v, u);
                     boundary_eta_point(a, nr - 1, v, u);
                 } else if (fam == 1 && u < np && v >= 1 && v < nr - 1) {
@@ -131,7 +132,7 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
     sspt::build_tables(hb, R1, th1, ph1, tab);
     sspt::build_fused_tables(hb, tab, ftab);
     const size_t n1 = nr + 1, n2 = nt + 1;
-    TabsG t;
+    TabsG<double> t;
     t.tr = ftab.data();
     t.tt = t.tr + sspt::FT_NTR * n1;
     t.t2 = t.tt + sspt::FT_NTT * n2;
@@ -144,7 +145,7 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
     int which = 0;
     for (int it = 0; it < niter; ++it) {
         const bool last = it == niter - 1, check = last && check_last;
-        Args a;
+        Args<double> a{};
         a.b = BoxG{nr, nt, np, hb.sj, hb.si};
         a.t = t;
         a.VR = in[SS_VR]; a.VT = in[SS_VTH]; a.VP = in[SS_VPH]; a.eta = in[SS_ETA];

@@ -1,0 +1,99 @@
+"""GPU: FP32 PT sweeps inside the FP64 defect-correction loop (and the FP64
+residual that measures time-to-tolerance).
+
+* k FP32 correction sweeps from a consistent FP64 state equal k plain FP64
+  iterations of the same (linear) problem up to FP32 rounding of the
+  increments: the PT map is affine and the correction sweep is its linear
+  part with the FP64 residual as forcing.
+* The mixed solve reaches the plain FP64 solution (same discrete equations)
+  to 1e-8 relative in V and P, for a linear and a power-law rheology.
+* The residual of an oracle-converged state is small, of a perturbed one not.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+from oracle.binding import Oracle
+from sphericalstokes_amd import setups
+from sphericalstokes_amd.fields import BC_FREE_SLIP, BC_SHEAR, FIELDS
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def Solver():
+    from sphericalstokes_amd.solver import Solver as S
+    return S
+
+
+def _load(S, st, fields=None):
+    s = S(st.nr, st.nth, st.nph)
+    s.set_grid(st.R, st.th, st.ph)
+    f = fields if fields is not None else st.fields
+    for n in FIELDS:
+        s.upload(n, f[n])
+    return s
+
+
+def test_correction_sweeps_equal_plain_iterations(Solver):
+    st = setups.diapir_sector(33, n_exp=1.0, bc_mode=BC_SHEAR, sh=0.5)
+    setups.random_state(st, seed=3, scale=1e-2)
+    st.fields["eta"][...] = st.fields["eta_ini"]          # n = 1: eta stays eta_ini
+    p = st.params
+    with _load(Solver, st) as a:
+        a.pt_solve_mixed(p, max_outer=1, max_inner=1, tol=np.inf)   # BCs + consistent tau only
+        base = a.download_all()
+    k = 7
+    with _load(Solver, st, base) as plain:
+        plain.pt_iterate(p, k)
+        ref = plain.download_all()
+    with _load(Solver, st, base) as mixed:
+        r = mixed.pt_solve_mixed(p, max_outer=1, max_inner=k, nout_inner=1000, tol=0.0)
+        got = mixed.download_all()
+    say("mixed one outer step", r)
+    for n in ("VR", "Vth", "Vph", "P"):
+        inc_ref = ref[n] - base[n]
+        inc = got[n] - base[n]
+        assert rel_l2(inc, inc_ref) < 1e-5, (n, rel_l2(inc, inc_ref))
+
+
+@pytest.mark.parametrize("n_exp", [1.0, 3.0])
+def test_mixed_reaches_fp64_solution(Solver, n_exp):
+    st = setups.diapir_sector(31, n_exp=n_exp, bc_mode=BC_FREE_SLIP, vr=1e2)
+    p = st.params
+    with _load(Solver, st) as s64:
+        r64 = s64.pt_solve_residual(p, 400000, 500, 1e-11)
+        ref = s64.download_all()
+    say("fp64", r64)
+    assert r64["converged"]
+    with _load(Solver, st) as sm:
+        rm = sm.pt_solve_mixed(p, max_outer=40, max_inner=20000, nout_inner=200, tol_inner=1e-4, tol=1e-11)
+        got = sm.download_all()
+        r_final = sm.residual(p)
+    say("mixed", {k: v for k, v in rm.items() if k != "r_hist"}, "r_hist", ["%.2e" % x for x in rm["r_hist"]])
+    assert rm["converged"] and r_final[0] <= 1e-11
+    for n in ("VR", "Vth", "Vph", "P"):
+        e = rel_l2(got[n], ref[n])
+        say(n, "rel error vs FP64 solution", e)
+        assert e < 1e-8, (n, e)
+
+
+def test_residual_small_at_oracle_fixed_point(Solver):
+    st = setups.diapir_sector(15, n_exp=1.0, bc_mode=BC_FREE_SLIP)
+    o = Oracle(st)
+    o.step(60000)
+    with _load(Solver, st, o.f) as s:
+        r = s.residual(st.params)
+        say("residual at the oracle's converged state", r)
+        assert r[0] < 1e-9
+        f = dict(o.f)
+        f["VR"] = f["VR"] + 1e-3 * np.abs(f["VR"]).max()
+        s.upload("VR", f["VR"])
+        r2 = s.residual(st.params)
+        assert r2[0] > 1e-6
