@@ -206,8 +206,10 @@ static int pick_rchunk(int nr, int tiles_xy)
         const int v = std::atoi(e);
         if (v >= 1) return std::min(v, nr);
     }
+    // about 4096 workgroups; a chunk keeps at least 8 planes (each chunk
+    // recomputes 2 warm-up planes), so small boxes still split the r-march
     int nz = (4096 + tiles_xy - 1) / tiles_xy;
-    nz = std::max(1, std::min(nz, nr / 32));
+    nz = std::max(1, std::min(nz, nr / 8));
     return (nr + nz - 1) / nz;
 }
 

@@ -172,7 +172,8 @@ template <typename R> struct Thread {
     R gtp, etp;                         // 4-point G / eta averages for tau_thph(s)
     // values of the current step
     R oP, oRR, oTT, oPP, oRTold, oRPold, oTPold, orho, oini, oPL;
-    R ogP, ofR, ofT, ofP;               // forcing of a correction sweep
+    R ogP, ofR, ofT, ofP;               // forcing of a correction sweep (step s+1 after phase 2)
+    R cfT, cfP;                         // th/ph forcing of step s, kept for phase 4
     R nRR, nTT, nPP, nSRR, nSTT, nSPP, nDT;
     R nrt, nrp, ntp;
     R pG_rt2, pE_rt2, pG_rp2, pE_rp2, gtp2, etp2;
@@ -196,6 +197,7 @@ template <typename R> SSPT_DH inline void thread_init(Thread<R> &T, const Args<R
     T.rt0 = T.rt0n = T.rp0 = T.rp0n = R(0);
     T.pG_rt = T.pE_rt = T.pG_rp = T.pE_rp = T.gtp = T.etp = R(1);
     T.nrt = T.nrp = T.ntp = R(0);
+    T.ogP = T.ofR = T.ofT = T.ofP = T.cfT = T.cfP = R(0);
 }
 
 // Row/column ring of the eta plane p for the star of the block's edge threads.
@@ -435,6 +437,10 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         }
     }
     T.crho = T.orho;
+    if (FL & F_CORR) {
+        T.cfT = T.ofT;
+        T.cfP = T.ofP;
+    }
     load_centre_old<R, FL>(T, a, s + 1);
 }
 
@@ -508,7 +514,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
                                 (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_y] * (tpK - T.ntp)) * a._dph) +
                                div_rcp(R(3) * aRT, Ry, iRy)) +
                               div_rcp(sTT - sPP, Ry, iRy) * t.tt[T.j * NTT + TT_cotf2];
-            if (FL & F_CORR) dV = dV + T.ofT;
+            if (FL & F_CORR) dV = dV + T.cfT;
             const R dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * R(0.5);
             const R v = T.vt0 + dV * dt;
             const long long o = b.off(s, T.j, T.k);
@@ -533,7 +539,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
                                 (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_z] * (T.nSPP - sPPm)) * a._dph) +
                                div_rcp(R(3) * aRP, Ry, iRy)) +
                               div_rcp(R(2) * aTP, Ry, iRy) * t.tt[T.j * NTT + TT_cotc2];
-            if (FL & F_CORR) dV = dV + T.ofP;
+            if (FL & F_CORR) dV = dV + T.cfP;
             const R dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * R(0.5);
             const R v = T.vp0 + dV * dt;
             const long long o = b.off(s, T.j, T.k);

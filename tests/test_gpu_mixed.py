@@ -78,10 +78,18 @@ def test_mixed_reaches_fp64_solution(Solver, n_exp):
         r_final = sm.residual(p)
     say("mixed", {k: v for k, v in rm.items() if k != "r_hist"}, "r_hist", ["%.2e" % x for x in rm["r_hist"]])
     assert rm["converged"] and r_final[0] <= 1e-11
-    for n in ("VR", "Vth", "Vph", "P"):
+    for n in ("VR", "Vth", "Vph"):
         e = rel_l2(got[n], ref[n])
         say(n, "rel error vs FP64 solution", e)
         assert e < 1e-8, (n, e)
+    # free slip on every face: a uniform pressure has zero gradient in every
+    # momentum stencil, so P is fixed up to a constant whose value depends on
+    # the iteration path; compare the parts with zero mean
+    dp = got["P"] - ref["P"]
+    pc = ref["P"] - ref["P"].mean()
+    e = float(np.linalg.norm(dp - dp.mean()) / np.linalg.norm(pc))
+    say("P (zero-mean part) rel error vs FP64 solution", e, "constant offset", float(dp.mean()))
+    assert e < 1e-8, ("P", e)
 
 
 def test_residual_small_at_oracle_fixed_point(Solver):
