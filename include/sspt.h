@@ -81,6 +81,9 @@ typedef struct ss_result {
 } ss_result;
 
 /* ---- context, grid, data movement ---- */
+/* One block of nr x nth x nph cells on one device.  Every field is stored in a
+ * box of (nr+1) x (nth+1) x roundup(nph+1, 64) doubles, which must stay below
+ * 4 GiB (about 775^3 cells); larger blocks return SS_ERR_ARG. */
 int ss_create(int nr, int nth, int nph, int device, ss_ctx **out);
 int ss_destroy(ss_ctx *ctx);
 const char *ss_last_error(const ss_ctx *ctx);

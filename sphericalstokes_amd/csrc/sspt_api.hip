@@ -156,6 +156,8 @@ int ss_create(int nr, int nth, int nph, int device, ss_ctx **out)
     b.sj = b.pitch;
     b.si = (long long)(nth + 1) * b.pitch;
     c->box_elems = (size_t)(nr + 1) * (size_t)b.si;
+    // the fused kernels address a field with 32-bit byte offsets
+    if (c->box_elems * sizeof(double) > 0xFFFFFFFFull) { delete c; return SS_ERR_ARG; }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return (int)e; }
     int rc = alloc_set(c, c->cur, nullptr);

@@ -8,9 +8,10 @@
 // faces of V (the BC_R!/BC_th!/BC_ph! sequence) and the boundary-cell
 // viscosities (power law of the copied tII).
 //
-// Block = 64 lanes along phi x 8 rows along th.  Lane tx covers k = kb-1+tx,
-// row ty covers j = jb-1+ty; the 62 x 6 inner points are owned (written),
-// lane 0 / row 0 are a low halo and lane 63 / row 7 a high halo that are
+// Block = 64 lanes along phi x 12 rows along th (12 wavefronts, 3 per SIMD).
+// Lane tx covers k = kb-1+tx, row ty covers j = jb-1+ty; the 62 x 10 inner
+// points are owned (written), lane 0 / row 0 are a low halo and lane 63 /
+// row 11 a high halo that are
 // recomputed from the old state.  The block marches along r: step s handles
 // centre plane s, r-face s (VR) and r-face s+1 (tau_Rth, tau_Rph).  Values of
 // the current plane that neighbours need go through LDS; values the same
@@ -36,7 +37,7 @@ namespace sspt {
 namespace fused {
 
 constexpr int TK = 64;        // lanes along phi (one wavefront)
-constexpr int TJB = 8;        // rows along th
+constexpr int TJB = 12;       // rows along th
 constexpr int OK_ = TK - 2;   // owned points per row
 constexpr int OJ = TJB - 2;   // owned rows
 constexpr double PI_ = 3.14159265358979323846;
@@ -50,7 +51,25 @@ struct BoxG {
     int nr, nt, np;
     long long sj, si;
     SSPT_DH long long off(int i, int j, int k) const { return (long long)i * si + (long long)j * sj + k; }
+    // the same offset in 32 bits; a field box is below 4 GiB (checked when
+    // the context is created), so byte offsets fit too
+    SSPT_DH uint32_t o32(int i, int j, int k) const
+    {
+        return (uint32_t)i * (uint32_t)si + (uint32_t)j * (uint32_t)sj + (uint32_t)k;
+    }
 };
+
+// Element o of a field through a 32-bit byte offset from the field's base:
+// with a wave-uniform base this is the SGPR-base + 32-bit VGPR-offset form of
+// global memory instructions, with no 64-bit address arithmetic per access.
+template <typename T> SSPT_DH inline T &fat(T *p, uint32_t o)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + (uint32_t)(o * (uint32_t)sizeof(T)));
+}
+template <typename T> SSPT_DH inline const T &fat(const T *p, uint32_t o)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint32_t)(o * (uint32_t)sizeof(T)));
+}
 
 // Interleaved FP64 metric tables (one record per index, see
 // build_fused_tables in sspt_tables.h): a record is read with a few scalar
@@ -206,48 +225,48 @@ template <typename R> SSPT_DH inline void load_ring(Thread<R> &T, const Args<R> 
     const BoxG &b = a.b;
     const int ie = clampi(p, 0, b.nr - 1);
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
-    if (T.ty == 0) T.ringJ = a.eta[b.off(ie, clampi(T.j - 1, 0, b.nt - 1), kcc)];
-    if (T.ty == TJB - 1) T.ringJ = a.eta[b.off(ie, clampi(T.j + 1, 0, b.nt - 1), kcc)];
-    if (T.tx == 0) T.ringK = a.eta[b.off(ie, jcc, clampi(T.k - 1, 0, b.np - 1))];
-    if (T.tx == TK - 1) T.ringK = a.eta[b.off(ie, jcc, clampi(T.k + 1, 0, b.np - 1))];
+    if (T.ty == 0) T.ringJ = fat(a.eta, b.o32(ie, clampi(T.j - 1, 0, b.nt - 1), kcc));
+    if (T.ty == TJB - 1) T.ringJ = fat(a.eta, b.o32(ie, clampi(T.j + 1, 0, b.nt - 1), kcc));
+    if (T.tx == 0) T.ringK = fat(a.eta, b.o32(ie, jcc, clampi(T.k - 1, 0, b.np - 1)));
+    if (T.tx == TK - 1) T.ringK = fat(a.eta, b.o32(ie, jcc, clampi(T.k + 1, 0, b.np - 1)));
 }
 
-template <typename R> SSPT_DH inline long long own_centre(const Thread<R> &T, const BoxG &b, int s)
+template <typename R> SSPT_DH inline uint32_t own_centre(const Thread<R> &T, const BoxG &b, int s)
 {
-    return b.off(clampi(s, 0, b.nr - 1), clampi(T.jc, 0, b.nt - 1), clampi(T.kc, 0, b.np - 1));
+    return b.o32(clampi(s, 0, b.nr - 1), clampi(T.jc, 0, b.nt - 1), clampi(T.kc, 0, b.np - 1));
 }
 
 // Own old values of step s.  They are issued one step ahead (right after
 // their previous use), so their latency hides behind a whole step.
 template <typename R, int FL> SSPT_DH inline void load_centre_old(Thread<R> &T, const Args<R> &a, int s)
 {
-    const long long oc = own_centre(T, a.b, s);
-    T.oP = a.P[oc];
-    T.oRR = a.tRR[oc];
-    T.oTT = a.tTT[oc];
-    T.oPP = a.tPP[oc];
+    const uint32_t oc = own_centre(T, a.b, s);
+    T.oP = fat(a.P, oc);
+    T.oRR = fat(a.tRR, oc);
+    T.oTT = fat(a.tTT, oc);
+    T.oPP = fat(a.tPP, oc);
     if (FL & F_CORR) {
-        T.ogP = a.gP[oc];
-        T.ofR = a.fVR[oc];
-        T.ofT = a.fVT[oc];
-        T.ofP = a.fVP[oc];
+        T.ogP = fat(a.gP, oc);
+        T.ofR = fat(a.fVR, oc);
+        T.ofT = fat(a.fVT, oc);
+        T.ofP = fat(a.fVP, oc);
     } else {
-        T.orho = a.rho[oc];
+        T.orho = fat(a.rho, oc);
     }
-    T.oTPold = a.tTP[a.b.off(clampi(s, 0, a.b.nr - 1), T.jc, T.kc)];
+    T.oTPold = fat(a.tTP, a.b.o32(clampi(s, 0, a.b.nr - 1), T.jc, T.kc));
 }
 template <typename R> SSPT_DH inline void load_face_old(Thread<R> &T, const Args<R> &a, int s)
 {
-    const long long oe = a.b.off(clampi(s + 1, 0, a.b.nr), T.jc, T.kc);
-    T.oRTold = a.tRT[oe];
-    T.oRPold = a.tRP[oe];
+    const uint32_t oe = a.b.o32(clampi(s + 1, 0, a.b.nr), T.jc, T.kc);
+    T.oRTold = fat(a.tRT, oe);
+    T.oRPold = fat(a.tRP, oe);
 }
 template <typename R, int FL> SSPT_DH inline void load_visc_old(Thread<R> &T, const Args<R> &a, int s)
 {
     if (!(FL & F_CORR) && T.own) {
-        const long long oc = own_centre(T, a.b, s);
-        T.oini = a.ini[oc];
-        T.oPL = (FL & F_RELAX) ? a.ePL[oc] : R(0);
+        const uint32_t oc = own_centre(T, a.b, s);
+        T.oini = fat(a.ini, oc);
+        T.oPL = (FL & F_RELAX) ? fat(a.ePL, oc) : R(0);
     }
 }
 
@@ -257,15 +276,15 @@ template <typename R, int FL> SSPT_DH inline void thread_prologue(Thread<R> &T, 
 {
     const BoxG &b = a.b;
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
-    T.e0 = a.eta[b.off(clampi(start, 0, b.nr - 1), jcc, kcc)];
-    T.e1 = a.eta[b.off(clampi(start + 1, 0, b.nr - 1), jcc, kcc)];
-    T.ne = a.eta[b.off(clampi(start + 2, 0, b.nr - 1), jcc, kcc)];
-    T.vr1 = a.VR[b.off(clampi(start, 0, b.nr), T.jc, T.kc)];
-    T.vt1 = a.VT[b.off(clampi(start, 0, b.nr - 1), T.jc, T.kc)];
-    T.vp1 = a.VP[b.off(clampi(start, 0, b.nr - 1), T.jc, T.kc)];
-    T.nvr = a.VR[b.off(clampi(start + 1, 0, b.nr), T.jc, T.kc)];
-    T.nvt = a.VT[b.off(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc)];
-    T.nvp = a.VP[b.off(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc)];
+    T.e0 = fat(a.eta, b.o32(clampi(start, 0, b.nr - 1), jcc, kcc));
+    T.e1 = fat(a.eta, b.o32(clampi(start + 1, 0, b.nr - 1), jcc, kcc));
+    T.ne = fat(a.eta, b.o32(clampi(start + 2, 0, b.nr - 1), jcc, kcc));
+    T.vr1 = fat(a.VR, b.o32(clampi(start, 0, b.nr), T.jc, T.kc));
+    T.vt1 = fat(a.VT, b.o32(clampi(start, 0, b.nr - 1), T.jc, T.kc));
+    T.vp1 = fat(a.VP, b.o32(clampi(start, 0, b.nr - 1), T.jc, T.kc));
+    T.nvr = fat(a.VR, b.o32(clampi(start + 1, 0, b.nr), T.jc, T.kc));
+    T.nvt = fat(a.VT, b.o32(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc));
+    T.nvp = fat(a.VP, b.o32(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc));
     T.ringJ = T.ringK = R(0);
     load_ring(T, a, start + 1);
     load_centre_old<R, FL>(T, a, start);
@@ -299,10 +318,10 @@ SSPT_DH inline void phase1(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s)
     if (T.tx == TK - 1) L.ETA[T.ty + 1][TK + 1] = T.ringK;
     // prefetch for step s+1
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
-    T.nvr = a.VR[b.off(clampi(s + 2, 0, nr), T.jc, T.kc)];
-    T.nvt = a.VT[b.off(clampi(s + 2, 0, nr - 1), T.jc, T.kc)];
-    T.nvp = a.VP[b.off(clampi(s + 2, 0, nr - 1), T.jc, T.kc)];
-    T.ne = a.eta[b.off(clampi(s + 3, 0, nr - 1), jcc, kcc)];
+    T.nvr = fat(a.VR, b.o32(clampi(s + 2, 0, nr), T.jc, T.kc));
+    T.nvt = fat(a.VT, b.o32(clampi(s + 2, 0, nr - 1), T.jc, T.kc));
+    T.nvp = fat(a.VP, b.o32(clampi(s + 2, 0, nr - 1), T.jc, T.kc));
+    T.ne = fat(a.eta, b.o32(clampi(s + 3, 0, nr - 1), jcc, kcc));
     load_ring(T, a, s + 2);
 }
 
@@ -334,7 +353,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         } else {
             const int dty = cj - T.j, dtx = ck - T.k;
             const int ty = clampi(T.ty + dty, 0, TJB - 1), tx = clampi(T.tx + dtx, 0, TK - 1);
-            const R em = a.eta[b.off(p - 1, cj, ck)], ep = a.eta[b.off(p + 1, cj, ck)];
+            const R em = fat(a.eta, b.o32(p - 1, cj, ck)), ep = fat(a.eta, b.o32(p + 1, cj, ck));
             T.sm1 = star(L, ty, tx, em, ep);
         }
     } else if (p == nr - 1) {
@@ -371,14 +390,14 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         T.nSTT = -nP + T.nTT;
         T.nSPP = -nP + T.nPP;
         if (wr && T.own) {
-            const long long o = b.off(s, T.j, T.k);
-            a.oP[o] = nP;
-            a.oRR[o] = T.nRR;
-            a.oTT[o] = T.nTT;
-            a.oPP[o] = T.nPP;
+            const uint32_t o = b.o32(s, T.j, T.k);
+            fat(a.oP, o) = nP;
+            fat(a.oRR, o) = T.nRR;
+            fat(a.oTT, o) = T.nTT;
+            fat(a.oPP, o) = T.nPP;
             if ((FL & F_LAST) && !(FL & F_CORR)) {
-                a.DT[o] = T.nDT;
-                a.SM[o] = T.sm0;
+                fat(a.DT, o) = T.nDT;
+                fat(a.SM, o) = T.sm0;
             }
         }
     } else {
@@ -399,7 +418,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         const R D = ((t.tr[s * NTR + TR_invRc4] * dVP) * a._dth + (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c4] * dVT) * a._dph) -
                          div_rcp(avVP, t.tr[s * NTR + TR_Rc4], t.tr[s * NTR + TR_invRc4]) * t.tt[T.j * NTT + TT_cotf4];
         T.ntp = (T.oTPold + T.gtp * D) / (R(1) + T.gtp / T.etp);
-        if (wr && T.own) a.oTP[b.off(s, T.j, T.k)] = T.ntp;
+        if (wr && T.own) fat(a.oTP, b.o32(s, T.j, T.k)) = T.ntp;
     }
     L.TP[T.ty][T.tx] = T.ntp;
 
@@ -424,16 +443,16 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         }
         const R dt = (T.cDT + T.nDT) * R(0.5);
         const R v = T.vr0 + dV * dt;
-        const long long o = b.off(s, T.j, T.k);
-        a.oVR[o] = v;
+        const uint32_t o = b.o32(s, T.j, T.k);
+        fat(a.oVR, o) = v;
         if (FL & F_CHECK) {
             const R d = v - T.vr0;
-            if (!(FL & F_CORR)) a.VRe[o] = d;
+            if (!(FL & F_CORR)) fat(a.VRe, o) = d;
             const unsigned long long ud = absbits(d), uv = absbits(v);
             T.mdv[0] = ud > T.mdv[0] ? ud : T.mdv[0];
             T.mv[0] = uv > T.mv[0] ? uv : T.mv[0];
         } else if ((FL & F_LAST) && !(FL & F_CORR)) {
-            a.VRe[o] = T.vr0;
+            fat(a.VRe, o) = T.vr0;
         }
     }
     T.crho = T.orho;
@@ -479,7 +498,7 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R D = ((T.vt1 - T.vt0) * a._dr + (iRe * (T.vr1 - L.VR[T.ty - 1][T.tx])) * a._dth) -
                              div_rcp((T.vt0 + T.vt1) * R(0.5), Re, iRe);
             T.nrt = (T.oRTold + G * D) / (R(1) + G / E);
-            if (wr && T.own) a.oRT[b.off(f, T.j, T.k)] = T.nrt;
+            if (wr && T.own) fat(a.oRT, b.o32(f, T.j, T.k)) = T.nrt;
         }
         if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
             const R G = ((T.pG_rp + gs * L.SM[T.ty][T.tx - 1]) + gs * T.sm1) * R(0.25);
@@ -487,7 +506,7 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R D = ((T.vp1 - T.vp0) * a._dr + (t.t2[(f * (nt + 1) + T.j) * NT2 + T2_e] * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
                              div_rcp((T.vp0 + T.vp1) * R(0.5), Re, iRe);
             T.nrp = (T.oRPold + G * D) / (R(1) + G / E);
-            if (wr && T.own) a.oRP[b.off(f, T.j, T.k)] = T.nrp;
+            if (wr && T.own) fat(a.oRP, b.o32(f, T.j, T.k)) = T.nrp;
         }
     }
     L.RT[T.ty][T.tx] = T.nrt;
@@ -517,16 +536,16 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             if (FL & F_CORR) dV = dV + T.cfT;
             const R dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * R(0.5);
             const R v = T.vt0 + dV * dt;
-            const long long o = b.off(s, T.j, T.k);
-            a.oVT[o] = v;
+            const uint32_t o = b.o32(s, T.j, T.k);
+            fat(a.oVT, o) = v;
             if (FL & F_CHECK) {
                 const R d = v - T.vt0;
-                if (!(FL & F_CORR)) a.VTe[o] = d;
+                if (!(FL & F_CORR)) fat(a.VTe, o) = d;
                 const unsigned long long ud = absbits(d), uv = absbits(v);
                 T.mdv[1] = ud > T.mdv[1] ? ud : T.mdv[1];
                 T.mv[1] = uv > T.mv[1] ? uv : T.mv[1];
             } else if ((FL & F_LAST) && !(FL & F_CORR)) {
-                a.VTe[o] = T.vt0;
+                fat(a.VTe, o) = T.vt0;
             }
         }
         // Vph at (s, j, k)
@@ -542,16 +561,16 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             if (FL & F_CORR) dV = dV + T.cfP;
             const R dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * R(0.5);
             const R v = T.vp0 + dV * dt;
-            const long long o = b.off(s, T.j, T.k);
-            a.oVP[o] = v;
+            const uint32_t o = b.o32(s, T.j, T.k);
+            fat(a.oVP, o) = v;
             if (FL & F_CHECK) {
                 const R d = v - T.vp0;
-                if (!(FL & F_CORR)) a.VPe[o] = d;
+                if (!(FL & F_CORR)) fat(a.VPe, o) = d;
                 const unsigned long long ud = absbits(d), uv = absbits(v);
                 T.mdv[2] = ud > T.mdv[2] ? ud : T.mdv[2];
                 T.mv[2] = uv > T.mv[2] ? uv : T.mv[2];
             } else if ((FL & F_LAST) && !(FL & F_CORR)) {
-                a.VPe[o] = T.vp0;
+                fat(a.VPe, o) = T.vp0;
             }
         }
         // tII and the new viscosity of interior cells (eta is frozen in a correction sweep)
@@ -563,10 +582,10 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
                 std::sqrt((((R(1) / R(2)) * ((T.nRR * T.nRR + T.nTT * T.nTT) + T.nPP * T.nPP) + x * x) + y * y) + z * z);
             R pl;
             const R e = power_law_pt(tii, T.oini, T.oPL, &pl, a, FL);
-            const long long o = b.off(s, T.j, T.k);
-            a.oEta[o] = e;
-            if (FL & F_RELAX) a.oPL[o] = pl;
-            if (FL & F_LAST) a.TII[o] = tii;
+            const uint32_t o = b.o32(s, T.j, T.k);
+            fat(a.oEta, o) = e;
+            if (FL & F_RELAX) fat(a.oPL, o) = pl;
+            if (FL & F_LAST) fat(a.TII, o) = tii;
         }
     }
     // carries for step s+1

@@ -50,9 +50,10 @@ def test_fused_matches_oracle_bitwise(kind, kw, niter):
     assert not bad, {k: res[k] for k in bad}
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 4), (7, 70, 5), (20, 9, 130), (9, 13, 63), (9, 13, 64)])
+@pytest.mark.parametrize("shape", [(4, 4, 4), (7, 70, 5), (20, 9, 130), (9, 13, 63), (9, 13, 64), (6, 10, 7),
+                                   (6, 11, 9), (5, 21, 66), (5, 31, 3)])
 def test_fused_tiling_edges(shape):
-    """Sizes around the 62 x 6 tile edges and the smallest grids."""
+    """Sizes around the 62 x 10 tile edges and the smallest grids."""
     s = _case("sector", n=shape, n_exp=1.0, bc_mode=BC_FREE_SLIP, seed=3)
     o = Oracle(s)
     o.step(2)
