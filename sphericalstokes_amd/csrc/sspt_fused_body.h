@@ -37,7 +37,10 @@ namespace sspt {
 namespace fused {
 
 constexpr int TK = 64;        // lanes along phi (one wavefront)
-constexpr int TJB = 12;       // rows along th
+#ifndef SSPT_TJB
+#define SSPT_TJB 12
+#endif
+constexpr int TJB = SSPT_TJB; // rows along th
 constexpr int OK_ = TK - 2;   // owned points per row
 constexpr int OJ = TJB - 2;   // owned rows
 constexpr double PI_ = 3.14159265358979323846;
@@ -77,6 +80,16 @@ template <typename T> SSPT_DH inline const T &fat(const T *p, uint32_t o)
 enum { TR_Rc, TR_invRc, TR_Rf, TR_invRf, TR_Re, TR_invRe, TR_Rc4, TR_invRc4, TR_Rc2, TR_invRc2, NTR = 16 };
 enum { TT_cotc, TT_cotf4, TT_cotc2, TT_cotf2, NTT = 4 };
 enum { T2_c, T2_e, T2_c4, T2_f, T2_y, T2_z, NT2 = 8 };
+
+// Table reads go through the constant address space: the compiler may then
+// assume that no store of the kernel changes them and, the index being
+// wave-uniform, read them with scalar loads.  Those are counted apart from
+// the vector memory operations, so a table read never waits for the
+// prefetches and stores of the field data in flight.
+#ifndef SSPT_CONSTANT_PTR
+#define SSPT_CONSTANT_PTR(T, p) ((const __attribute__((address_space(4))) T *)(p))
+#endif
+template <typename R> SSPT_DH inline R tload(const R *p, int i) { return SSPT_CONSTANT_PTR(R, p)[i]; }
 
 template <typename R> struct TabsG {
     const R *tr; // [i*NTR + TR_*], i in [0, nr]
@@ -365,13 +378,13 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
     const bool centre = T.j >= 0 && T.j < nt && T.k >= 0 && T.k < np;
     // C(s): compute_P! centre part
     if (s >= 0 && s <= nr - 1 && centre && T.tx < TK - 1 && T.ty < TJB - 1) {
-        const R Rv = t.tr[s * NTR + TR_Rc], iR = t.tr[s * NTR + TR_invRc];
+        const R Rv = tload(t.tr, s * NTR + TR_Rc), iR = tload(t.tr, s * NTR + TR_invRc);
         const R dVRr = (T.vr1 - T.vr0) * a._dr;
         const R dVTt = (iR * (L.VT[T.ty + 1][T.tx] - T.vt0)) * a._dth;
-        const R dVPp = (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c] * (L.VP[T.ty][T.tx + 1] - T.vp0)) * a._dph;
+        const R dVPp = (tload(t.t2, (s * (nt + 1) + T.j) * NT2 + T2_c) * (L.VP[T.ty][T.tx + 1] - T.vp0)) * a._dph;
         const R avVR = (T.vr0 + T.vr1) * R(0.5);
         const R avVT = (T.vt0 + L.VT[T.ty + 1][T.tx]) * R(0.5);
-        const R ct = t.tt[T.j * NTT + TT_cotc];
+        const R ct = tload(t.tt, T.j * NTT + TT_cotc);
         const R vr_R = div_rcp(avVR, Rv, iR), vt_R = div_rcp(avVT, Rv, iR);
         const R divV = (((dVRr + dVTt) + dVPp) + div_rcp(R(2) * avVR, Rv, iR)) + vt_R * ct;
         const R third = R(1) / R(3);
@@ -415,8 +428,8 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         const R dVP = T.vp0 - L.VP[T.ty - 1][T.tx];
         const R dVT = T.vt0 - L.VT[T.ty][T.tx - 1];
         const R avVP = (L.VP[T.ty - 1][T.tx] + T.vp0) * R(0.5);
-        const R D = ((t.tr[s * NTR + TR_invRc4] * dVP) * a._dth + (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_c4] * dVT) * a._dph) -
-                         div_rcp(avVP, t.tr[s * NTR + TR_Rc4], t.tr[s * NTR + TR_invRc4]) * t.tt[T.j * NTT + TT_cotf4];
+        const R D = ((tload(t.tr, s * NTR + TR_invRc4) * dVP) * a._dth + (tload(t.t2, (s * (nt + 1) + T.j) * NT2 + T2_c4) * dVT) * a._dph) -
+                         div_rcp(avVP, tload(t.tr, s * NTR + TR_Rc4), tload(t.tr, s * NTR + TR_invRc4)) * tload(t.tt, T.j * NTT + TT_cotf4);
         T.ntp = (T.oTPold + T.gtp * D) / (R(1) + T.gtp / T.etp);
         if (wr && T.own) fat(a.oTP, b.o32(s, T.j, T.k)) = T.ntp;
     }
@@ -424,16 +437,16 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
 
     // VR at r-face s (compute_V!): needs C(s-1), C(s), tau_Rth/tau_Rph of face s
     if (wr && T.own && s >= 1 && s <= nr - 1 && T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 2) {
-        const R Rx = t.tr[s * NTR + TR_Rf], iRx = t.tr[s * NTR + TR_invRf];
+        const R Rx = tload(t.tr, s * NTR + TR_Rf), iRx = tload(t.tr, s * NTR + TR_invRf);
         const R sRR = (T.cRR + T.nSRR) * R(0.5);
         const R sTT = (T.cTT + T.nSTT) * R(0.5);
         const R sPP = (T.cPP + T.nSPP) * R(0.5);
         const R aRT = (T.rt0 + T.rt0n) * R(0.5);
         const R dsig = (((((T.nSRR - T.cRR) * a._dr + (iRx * (T.rt0n - T.rt0)) * a._dth) +
-                            (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_f] * (T.rp0n - T.rp0)) * a._dph) +
+                            (tload(t.t2, (s * (nt + 1) + T.j) * NT2 + T2_f) * (T.rp0n - T.rp0)) * a._dph) +
                            div_rcp(R(2) * sRR, Rx, iRx)) -
                           div_rcp(sTT + sPP, Rx, iRx)) +
-                         div_rcp(aRT, Rx, iRx) * t.tt[T.j * NTT + TT_cotc2];
+                         div_rcp(aRT, Rx, iRx) * tload(t.tt, T.j * NTT + TT_cotc2);
         R dV;
         if (FL & F_CORR) {
             dV = dsig + T.ofR;
@@ -491,7 +504,7 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
     T.nrt = R(0);
     T.nrp = R(0);
     if (f >= 1 && f <= nr - 1 && inner) {
-        const R Re = t.tr[f * NTR + TR_Re], iRe = t.tr[f * NTR + TR_invRe];
+        const R Re = tload(t.tr, f * NTR + TR_Re), iRe = tload(t.tr, f * NTR + TR_invRe);
         if (T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 2) {
             const R G = ((T.pG_rt + gs * L.SM[T.ty - 1][T.tx]) + gs * T.sm1) * R(0.25);
             const R E = ((T.pE_rt + L.ETA[T.ty][T.tx + 1]) + T.e1) * R(0.25);
@@ -503,7 +516,7 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
             const R G = ((T.pG_rp + gs * L.SM[T.ty][T.tx - 1]) + gs * T.sm1) * R(0.25);
             const R E = ((T.pE_rp + L.ETA[T.ty + 1][T.tx]) + T.e1) * R(0.25);
-            const R D = ((T.vp1 - T.vp0) * a._dr + (t.t2[(f * (nt + 1) + T.j) * NT2 + T2_e] * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
+            const R D = ((T.vp1 - T.vp0) * a._dr + (tload(t.t2, (f * (nt + 1) + T.j) * NT2 + T2_e) * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
                              div_rcp((T.vp0 + T.vp1) * R(0.5), Re, iRe);
             T.nrp = (T.oRPold + G * D) / (R(1) + G / E);
             if (wr && T.own) fat(a.oRP, b.o32(f, T.j, T.k)) = T.nrp;
@@ -522,7 +535,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
     const TabsG<R> &t = a.t;
     const int nr = b.nr, nt = b.nt, np = b.np;
     if (wr && T.own && s >= 1 && s <= nr - 2) {
-        const R Ry = t.tr[s * NTR + TR_Rc2], iRy = t.tr[s * NTR + TR_invRc2];
+        const R Ry = tload(t.tr, s * NTR + TR_Rc2), iRy = tload(t.tr, s * NTR + TR_invRc2);
         // Vth at (s, j, k)
         if (T.j >= 1 && T.j <= nt - 1 && T.k >= 1 && T.k <= np - 2) {
             const R sTTm = L.STT[T.ty - 1][T.tx], sPPm = L.SPP[T.ty - 1][T.tx];
@@ -530,9 +543,9 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R aRT = (T.rt0 + T.nrt) * R(0.5);
             const R tpK = L.TP[T.ty][T.tx + 1];
             R dV = ((((T.nrt - T.rt0) * a._dr + (iRy * (T.nSTT - sTTm)) * a._dth) +
-                                (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_y] * (tpK - T.ntp)) * a._dph) +
+                                (tload(t.t2, (s * (nt + 1) + T.j) * NT2 + T2_y) * (tpK - T.ntp)) * a._dph) +
                                div_rcp(R(3) * aRT, Ry, iRy)) +
-                              div_rcp(sTT - sPP, Ry, iRy) * t.tt[T.j * NTT + TT_cotf2];
+                              div_rcp(sTT - sPP, Ry, iRy) * tload(t.tt, T.j * NTT + TT_cotf2);
             if (FL & F_CORR) dV = dV + T.cfT;
             const R dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * R(0.5);
             const R v = T.vt0 + dV * dt;
@@ -555,9 +568,9 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R aRP = (T.rp0 + T.nrp) * R(0.5);
             const R aTP = (T.ntp + tpJ) * R(0.5);
             R dV = ((((T.nrp - T.rp0) * a._dr + (iRy * (tpJ - T.ntp)) * a._dth) +
-                                (t.t2[(s * (nt + 1) + T.j) * NT2 + T2_z] * (T.nSPP - sPPm)) * a._dph) +
+                                (tload(t.t2, (s * (nt + 1) + T.j) * NT2 + T2_z) * (T.nSPP - sPPm)) * a._dph) +
                                div_rcp(R(3) * aRP, Ry, iRy)) +
-                              div_rcp(R(2) * aTP, Ry, iRy) * t.tt[T.j * NTT + TT_cotc2];
+                              div_rcp(R(2) * aTP, Ry, iRy) * tload(t.tt, T.j * NTT + TT_cotc2);
             if (FL & F_CORR) dV = dV + T.cfP;
             const R dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * R(0.5);
             const R v = T.vp0 + dV * dt;

@@ -8,6 +8,7 @@
 // CPU oracle without a GPU.  It is never part of the product library.
 #define SSPT_DH
 #define SSPT_UNIFORM(x) (x)
+#define SSPT_CONSTANT_PTR(T, p) (p)
 #include "../../sphericalstokes_amd/csrc/sspt_fused_body.h"
 #include "../../sphericalstokes_amd/csrc/sspt_tables.h"
 #include "../../include/sspt.h"
