@@ -33,15 +33,15 @@ def a_eff_bytes(n):
 
 def load_traffic(n):
     """HBM bytes per main-kernel launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    p = ROOT / "profiles" / f"fused_main_{n}_traffic.json"
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    its `bytes_per_launch:` line; None if there is no summary for this size."""
+    p = ROOT / "profiles" / f"fused_main_{n}_traffic.txt"
     if not p.exists():
         return None
-    try:
-        d = json.loads(p.read_text())
-        return float(d["bytes_per_launch"])
-    except Exception:
-        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(setup, n, iters):
