@@ -9,6 +9,12 @@ SphericalStokes_diapir.jl:447).
 
 Prints one JSON line.  The CPU baseline is the C/OpenMP oracle (a port of the
 reference CPU path) timed on a bounded sample of the same workload.
+
+"mixed_precision" holds the second half of the metric: the cost of one FP32
+correction sweep on the same 511^3 workload against one FP64 iteration, and
+the time to reach an FP64 residual of 1e-8 with the plain FP64 solve and
+with the FP32-inner / FP64 defect-correction solve (linear diapir_sector at
+--ttt-n^3, default 63^3 so that the default run stays within minutes).
 """
 from __future__ import annotations
 
@@ -57,6 +63,71 @@ def cpu_baseline(setup, n, iters):
                       f"iteration, C/OpenMP oracle (reference kernel sequence), t_it = {dt:.3f} s"}
 
 
+def fp32_sweep_ms(s, p, sweeps):
+    """Average wall time of one FP32 correction sweep on the same workload:
+    one outer step of the mixed solve with `sweeps` sweeps and no checks."""
+    r = s.pt_solve_mixed(p, max_outer=1, max_inner=sweeps, nout_inner=sweeps + 1, tol_inner=0.0, tol=0.0)
+    return r["inner_seconds"] / sweeps * 1e3
+
+
+def time_to_tolerance(n, tol, tol_inner):
+    """Time-to-tolerance of the plain FP64 solve and of the FP32-inner /
+    FP64 defect-correction solve on the linear diapir_sector at n^3 (free
+    slip: the discrete equations are consistent, so the residual can reach
+    tol).  Both start from the same state; the FP64 solve checks its residual
+    every 500 iterations.  Error of the mixed velocity against the FP64 one."""
+    import numpy as np
+    from sphericalstokes_amd import setups
+    from sphericalstokes_amd.fields import BC_FREE_SLIP, FIELDS
+    from sphericalstokes_amd.solver import Solver
+
+    st = setups.diapir_sector(n, n_exp=1.0, bc_mode=BC_FREE_SLIP)
+    p = st.params
+
+    def load():
+        s = Solver(n, n, n)
+        s.set_grid(st.R, st.th, st.ph)
+        for name in FIELDS:
+            s.upload(name, st.fields[name])
+        return s
+
+    # Both solves run in chunks (FP64: 20000 iterations, mixed: one outer
+    # step per call) so that progress is reported on stderr; a chunk ends at
+    # a residual check in either case, so the iterates are those of one call.
+    s = load()
+    t0 = time.perf_counter()
+    iters, conv64 = 0, False
+    while iters < 4_000_000 and not conv64:
+        r64 = s.pt_solve_residual(p, 20_000, 500, tol)
+        iters += r64["iter_end"] if r64["converged"] else r64["iters"]
+        conv64 = r64["converged"]
+        print(f"  fp64 {n}^3: {iters} iterations, residual {r64['r']:.3e}", file=sys.stderr, flush=True)
+    t64 = time.perf_counter() - t0
+    ref = {k: s.download(k) for k in ("VR", "Vth", "Vph")}
+    s.close()
+    s = load()
+    t0 = time.perf_counter()
+    outer, inner, hist, convm = 0, 0, [], False
+    while outer < 40 and not convm:
+        rm = s.pt_solve_mixed(p, max_outer=1, max_inner=4_000_000, nout_inner=200, tol_inner=tol_inner, tol=tol)
+        if not hist:
+            hist.append(rm["r_hist"][0])
+        convm = rm["converged"]
+        outer += rm["outer"]
+        inner += rm["inner_total"]
+        hist.append(rm["r"])
+        print(f"  mixed {n}^3: outer {outer}, {inner} FP32 sweeps, residual {rm['r']:.3e}", file=sys.stderr,
+              flush=True)
+    tm = time.perf_counter() - t0
+    err = max(float(np.linalg.norm(s.download(k) - ref[k]) / np.linalg.norm(ref[k])) for k in ref)
+    s.close()
+    return {"workload": f"diapir_sector {n}^3, n_exp=1, free slip, FP64 residual <= {tol:g}",
+            "fp64_s": round(t64, 3), "fp64_iters": iters, "fp64_converged": conv64,
+            "mixed_s": round(tm, 3), "mixed_outer": outer, "mixed_inner_sweeps": inner,
+            "mixed_converged": convm, "mixed_r_hist": [float("%.3e" % x) for x in hist],
+            "ratio_mixed_over_fp64": round(tm / t64, 4), "V_rel_err_mixed_vs_fp64": float("%.3e" % err)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +136,11 @@ def main():
     ap.add_argument("--n", type=int, default=511)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ttt-n", type=int, default=63,
+                    help="size of the time-to-tolerance comparison (FP64 vs FP32-refined); 0 skips it")
+    ap.add_argument("--ttt-tol", type=float, default=1e-8)
+    ap.add_argument("--fp32-sweeps", type=int, default=20,
+                    help="FP32 correction sweeps timed on the main workload; 0 skips them")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain field copy (k_copy_field: n^3 doubles read + written) so that "
                          "rocprofv3 --pmc byte counters can be scaled")
@@ -90,6 +166,7 @@ def main():
     t0 = time.perf_counter()
     main_ms, iter_ms = s.pt_iterate_timed(p, args.steps)
     wall = time.perf_counter() - t0
+    sweep_ms = fp32_sweep_ms(s, p, args.fp32_sweeps) if args.fp32_sweeps > 0 else None
     s.close()
     ms_per_step = wall / args.steps * 1e3
     A = a_eff_bytes(n)
@@ -118,7 +195,13 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": "k_fused_main", "kernel_ms": round(main_ms, 4)},
         "cpu_baseline": None,
+        "mixed_precision": {
+            "fp32_sweep_ms": round(sweep_ms, 4) if sweep_ms else None,
+            "fp32_sweep_over_fp64_iter": round(sweep_ms / ms_per_step, 4) if sweep_ms else None,
+            "time_to_tol": None},
     }
+    if args.ttt_n > 0:
+        out["mixed_precision"]["time_to_tol"] = time_to_tolerance(args.ttt_n, args.ttt_tol, 1e-5)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(st, n, args.cpu_iters)
     print(json.dumps(out), flush=True)

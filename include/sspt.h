@@ -165,6 +165,7 @@ typedef struct ss_mixed_result {
     double r_hist[SS_MIXED_MAX_OUTER + 1];   /* residual before each outer step (and final) */
     double err_inner_hist[SS_MIXED_MAX_OUTER];
     double r, seconds;
+    double inner_seconds;   /* part of seconds spent in the FP32 sweeps */
 } ss_mixed_result;
 /* FP32 PT sweeps inside an FP64 defect-correction loop (no reference
  * counterpart).  Each outer step resets tau to the stress consistent with V

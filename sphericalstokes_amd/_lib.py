@@ -49,7 +49,8 @@ class ss_mixed_opts(C.Structure):
 class ss_mixed_result(C.Structure):
     _fields_ = [("outer", C.c_int), ("inner_total", C.c_int), ("converged", C.c_int),
                 ("inner_hist", C.c_int * MIXED_MAX_OUTER), ("r_hist", C.c_double * (MIXED_MAX_OUTER + 1)),
-                ("err_inner_hist", C.c_double * MIXED_MAX_OUTER), ("r", C.c_double), ("seconds", C.c_double)]
+                ("err_inner_hist", C.c_double * MIXED_MAX_OUTER), ("r", C.c_double), ("seconds", C.c_double),
+                ("inner_seconds", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/sspt.h

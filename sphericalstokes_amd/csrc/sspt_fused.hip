@@ -50,13 +50,13 @@ __device__ __forceinline__ void tile_of(int nx, int ny, int nz, int &bx, int &by
 }
 
 template <typename R, int FL>
-__global__ void __launch_bounds__(TK * TJB) k_fused_main(Args<R> a, int nx, int ny, int nz)
+__global__ void __launch_bounds__(TK * TJB<R>) k_fused_main(Args<R> a, int nx, int ny, int nz)
 {
     __shared__ Lds<R> L;
     int bx, by, bz;
     tile_of(nx, ny, nz, bx, by, bz);
     if (bz >= nz) return; // padding slots of the remap (uniform per block)
-    const int kb = bx * OK_, jb = by * OJ;
+    const int kb = bx * OK_, jb = by * OJ<R>;
     const int s0 = bz * a.rchunk, s1 = min(s0 + a.rchunk, a.b.nr);
     const int start = max(s0 - 2, -1);
     Thread<R> T;
@@ -206,10 +206,11 @@ static int pick_rchunk(int nr, int tiles_xy)
         const int v = std::atoi(e);
         if (v >= 1) return std::min(v, nr);
     }
-    // about 4096 workgroups; a chunk keeps at least 8 planes (each chunk
+    // about 4096 workgroups; a chunk keeps at least 4 planes (each chunk
     // recomputes 2 warm-up planes), so small boxes still split the r-march
+    // (measured at 63^3 and 127^3: chunks of 4-6 planes are fastest there)
     int nz = (4096 + tiles_xy - 1) / tiles_xy;
-    nz = std::max(1, std::min(nz, nr / 8));
+    nz = std::max(1, std::min(nz, nr / 4));
     return (nr + nz - 1) / nz;
 }
 
@@ -219,14 +220,14 @@ template <typename R>
 void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_main1)
 {
     const int nx = (a.b.np + OK_ - 1) / OK_;
-    const int ny = (a.b.nt + OJ - 1) / OJ;
+    const int ny = (a.b.nt + OJ<R> - 1) / OJ<R>;
     a.rchunk = pick_rchunk(a.b.nr, nx * ny);
     const int nz = (a.b.nr + a.rchunk - 1) / a.rchunk;
     if (a.flags & F_CHECK) hipLaunchKernelGGL(k_zero_red, dim3(1), dim3(64), 0, c->stream, a.red);
     const int T = nx * ny * nz;
     const int grid = ((T + 7) / 8) * 8;
     if (ev_main0) (void)hipEventRecord(ev_main0, c->stream);
-    const dim3 G(grid), B(TK, TJB);
+    const dim3 G(grid), B(TK, TJB<R>);
     if constexpr (sizeof(R) == 8) {
         switch (a.flags) {
 #define SSPT_FL(v) case v: hipLaunchKernelGGL((k_fused_main<R, v>), G, B, 0, c->stream, a, nx, ny, nz); break;

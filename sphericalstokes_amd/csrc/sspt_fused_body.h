@@ -8,10 +8,10 @@
 // faces of V (the BC_R!/BC_th!/BC_ph! sequence) and the boundary-cell
 // viscosities (power law of the copied tII).
 //
-// Block = 64 lanes along phi x 12 rows along th (12 wavefronts, 3 per SIMD).
-// Lane tx covers k = kb-1+tx, row ty covers j = jb-1+ty; the 62 x 10 inner
-// points are owned (written), lane 0 / row 0 are a low halo and lane 63 /
-// row 11 a high halo that are
+// Block = 64 lanes along phi x TJB rows along th (12 in FP64, 16 in FP32; one
+// wavefront per row).  Lane tx covers k = kb-1+tx, row ty covers j = jb-1+ty;
+// the 62 x (TJB-2) inner points are owned (written), lane 0 / row 0 are a low
+// halo and lane 63 / row TJB-1 a high halo that are
 // recomputed from the old state.  The block marches along r: step s handles
 // centre plane s, r-face s (VR) and r-face s+1 (tau_Rth, tau_Rph).  Values of
 // the current plane that neighbours need go through LDS; values the same
@@ -37,12 +37,17 @@ namespace sspt {
 namespace fused {
 
 constexpr int TK = 64;        // lanes along phi (one wavefront)
-#ifndef SSPT_TJB
-#define SSPT_TJB 12
+// rows along th: 12 for FP64 (3 wavefronts per SIMD at <= 168 VGPRs), 16 for
+// FP32 (4 per SIMD)
+#ifndef SSPT_TJB64
+#define SSPT_TJB64 12
 #endif
-constexpr int TJB = SSPT_TJB; // rows along th
+#ifndef SSPT_TJB32
+#define SSPT_TJB32 16
+#endif
+template <typename R> constexpr int TJB = sizeof(R) == 8 ? SSPT_TJB64 : SSPT_TJB32;
 constexpr int OK_ = TK - 2;   // owned points per row
-constexpr int OJ = TJB - 2;   // owned rows
+template <typename R> constexpr int OJ = TJB<R> - 2; // owned rows
 constexpr double PI_ = 3.14159265358979323846;
 
 // F_CORR: correction sweep of the FP32/FP64 defect-correction solve (forcing
@@ -119,11 +124,11 @@ template <typename R> struct Args {
 };
 
 template <typename R> struct Lds {
-    R VR[TJB][TK], VT[TJB][TK], VP[TJB][TK];
-    R ETA[TJB + 2][TK + 2];
-    R SM[TJB][TK];
-    R STT[TJB][TK], SPP[TJB][TK], DT[TJB][TK];
-    R RT[TJB][TK], RP[TJB][TK], TP[TJB][TK];
+    R VR[TJB<R>][TK], VT[TJB<R>][TK], VP[TJB<R>][TK];
+    R ETA[TJB<R> + 2][TK + 2];
+    R SM[TJB<R>][TK];
+    R STT[TJB<R>][TK], SPP[TJB<R>][TK], DT[TJB<R>][TK];
+    R RT[TJB<R>][TK], RP[TJB<R>][TK], TP[TJB<R>][TK];
 };
 
 // Julia Base.max/min for Float64
@@ -220,7 +225,7 @@ template <typename R> SSPT_DH inline void thread_init(Thread<R> &T, const Args<R
     T.j = SSPT_UNIFORM(jb - 1 + ty);
     T.kc = clampi(T.k, 0, a.b.np);
     T.jc = clampi(T.j, 0, a.b.nt);
-    T.own = tx >= 1 && tx <= TK - 2 && ty >= 1 && ty <= TJB - 2 && T.j < a.b.nt && T.k < a.b.np;
+    T.own = tx >= 1 && tx <= TK - 2 && ty >= 1 && ty <= TJB<R> - 2 && T.j < a.b.nt && T.k < a.b.np;
     for (int c = 0; c < 3; ++c) T.mdv[c] = T.mv[c] = 0ull;
     T.vr0 = T.vr1 = T.vt0 = T.vt1 = T.vp0 = T.vp1 = R(0);
     T.e0 = T.e1 = T.e2 = R(1);
@@ -239,7 +244,7 @@ template <typename R> SSPT_DH inline void load_ring(Thread<R> &T, const Args<R> 
     const int ie = clampi(p, 0, b.nr - 1);
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
     if (T.ty == 0) T.ringJ = fat(a.eta, b.o32(ie, clampi(T.j - 1, 0, b.nt - 1), kcc));
-    if (T.ty == TJB - 1) T.ringJ = fat(a.eta, b.o32(ie, clampi(T.j + 1, 0, b.nt - 1), kcc));
+    if (T.ty == TJB<R> - 1) T.ringJ = fat(a.eta, b.o32(ie, clampi(T.j + 1, 0, b.nt - 1), kcc));
     if (T.tx == 0) T.ringK = fat(a.eta, b.o32(ie, jcc, clampi(T.k - 1, 0, b.np - 1)));
     if (T.tx == TK - 1) T.ringK = fat(a.eta, b.o32(ie, jcc, clampi(T.k + 1, 0, b.np - 1)));
 }
@@ -326,7 +331,7 @@ SSPT_DH inline void phase1(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s)
     // eta plane s+1 with a one-point ring for the 7-point star of the halo
     L.ETA[T.ty + 1][T.tx + 1] = T.e1;
     if (T.ty == 0) L.ETA[0][T.tx + 1] = T.ringJ;
-    if (T.ty == TJB - 1) L.ETA[TJB + 1][T.tx + 1] = T.ringJ;
+    if (T.ty == TJB<R> - 1) L.ETA[TJB<R> + 1][T.tx + 1] = T.ringJ;
     if (T.tx == 0) L.ETA[T.ty + 1][0] = T.ringK;
     if (T.tx == TK - 1) L.ETA[T.ty + 1][TK + 1] = T.ringK;
     // prefetch for step s+1
@@ -365,7 +370,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             T.sm1 = star(L, T.ty, T.tx, T.e0, T.e2);
         } else {
             const int dty = cj - T.j, dtx = ck - T.k;
-            const int ty = clampi(T.ty + dty, 0, TJB - 1), tx = clampi(T.tx + dtx, 0, TK - 1);
+            const int ty = clampi(T.ty + dty, 0, TJB<R> - 1), tx = clampi(T.tx + dtx, 0, TK - 1);
             const R em = fat(a.eta, b.o32(p - 1, cj, ck)), ep = fat(a.eta, b.o32(p + 1, cj, ck));
             T.sm1 = star(L, ty, tx, em, ep);
         }
@@ -377,7 +382,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
 
     const bool centre = T.j >= 0 && T.j < nt && T.k >= 0 && T.k < np;
     // C(s): compute_P! centre part
-    if (s >= 0 && s <= nr - 1 && centre && T.tx < TK - 1 && T.ty < TJB - 1) {
+    if (s >= 0 && s <= nr - 1 && centre && T.tx < TK - 1 && T.ty < TJB<R> - 1) {
         const R Rv = tload(t.tr, s * NTR + TR_Rc), iR = tload(t.tr, s * NTR + TR_invRc);
         const R dVRr = (T.vr1 - T.vr0) * a._dr;
         const R dVTt = (iR * (L.VT[T.ty + 1][T.tx] - T.vt0)) * a._dth;
@@ -602,7 +607,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         }
     }
     // carries for step s+1
-    if (T.ty + 1 < TJB) T.rt0n = L.RT[T.ty + 1][T.tx];
+    if (T.ty + 1 < TJB<R>) T.rt0n = L.RT[T.ty + 1][T.tx];
     if (T.tx + 1 < TK) T.rp0n = L.RP[T.ty][T.tx + 1];
     T.rt0 = T.nrt;
     T.rp0 = T.nrp;

@@ -367,6 +367,8 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+constexpr int CHECK_PLANES = 32;
+
 __global__ void __launch_bounds__(256) k_check_err(Box b, double *__restrict__ VRe, double *__restrict__ VTe,
                                                    double *__restrict__ VPe, const double *__restrict__ VR,
                                                    const double *__restrict__ VT, const double *__restrict__ VP,
@@ -374,9 +376,10 @@ __global__ void __launch_bounds__(256) k_check_err(Box b, double *__restrict__ V
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
     unsigned long long me[3] = {0, 0, 0}, mv[3] = {0, 0, 0};
-    if (i <= b.nr && j <= b.nt && k <= b.np) {
+    // CHECK_PLANES r-planes per thread: one set of atomics per wavefront
+    const int i0 = blockIdx.z * CHECK_PLANES, i1 = min(i0 + CHECK_PLANES, b.nr + 1);
+    for (int i = i0; i < i1 && j <= b.nt && k <= b.np; ++i) {
         const long long o = b.off(i, j, k);
         const bool in[3] = {j < b.nt && k < b.np, i < b.nr && k < b.np, i < b.nr && j < b.nt};
         double *E[3] = {VRe, VTe, VPe};
@@ -387,8 +390,10 @@ __global__ void __launch_bounds__(256) k_check_err(Box b, double *__restrict__ V
             const double v = V[c][o];
             const double d = v - E[c][o];
             if (write_err) E[c][o] = d;
-            me[c] = (unsigned long long)__double_as_longlong(fabs(d));
-            mv[c] = (unsigned long long)__double_as_longlong(fabs(v));
+            const unsigned long long ud = (unsigned long long)__double_as_longlong(fabs(d));
+            const unsigned long long uv = (unsigned long long)__double_as_longlong(fabs(v));
+            me[c] = ud > me[c] ? ud : me[c];
+            mv[c] = uv > mv[c] ? uv : mv[c];
         }
     }
 #pragma unroll
@@ -615,7 +620,8 @@ void launch_check_err(ss_ctx *c, const FieldSet &s, const double *VRprev, const 
     const Box &b = c->box;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->red, 6);
     // when prev != V_err the kernel only reads (write_err = 0) or writes V_err
-    hipLaunchKernelGGL(k_check_err, grid3(b.nr + 1, b.nt + 1, b.np + 1, B3), B3, 0, c->stream, b,
+    hipLaunchKernelGGL(k_check_err, grid3((b.nr + 1 + CHECK_PLANES - 1) / CHECK_PLANES, b.nt + 1, b.np + 1, B3), B3, 0,
+                       c->stream, b,
                        const_cast<double *>(VRprev), const_cast<double *>(VTprev), const_cast<double *>(VPprev),
                        s.f[SS_VR], s.f[SS_VTH], s.f[SS_VPH], c->red, write_err);
 }

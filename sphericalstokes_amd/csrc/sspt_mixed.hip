@@ -40,6 +40,9 @@ __device__ __forceinline__ unsigned long long wmax(unsigned long long v)
     return v;
 }
 
+__device__ __forceinline__ unsigned long long umax(unsigned long long x, unsigned long long y) { return x > y ? x : y; }
+constexpr int RES_PLANES = 32;
+
 #define M_LOOP(NI, NJ, NK)                                        \
     const int k = blockIdx.x * blockDim.x + threadIdx.x;          \
     const int j = blockIdx.y * blockDim.y + threadIdx.y;          \
@@ -111,10 +114,13 @@ __global__ void __launch_bounds__(256) k_residual(Box b, Tables t, ResArgs a)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
     const int nr = b.nr, nt = b.nt, np = b.np;
     unsigned long long m[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (i <= nr && j <= nt && k <= np) {
+    // each thread marches RES_PLANES planes and keeps running maxima, so
+    // that a wavefront issues its atomics once, not once per plane
+    const int i0 = blockIdx.z * RES_PLANES, i1 = min(i0 + RES_PLANES, nr + 1);
+    for (int i = i0; i < i1; ++i) {
+        if (j > nt || k > np) break;
         const long long o = b.off(i, j, k);
         if (i < nr && j < nt && k < np) { // centre
             const double R = t.Rc[i];
@@ -125,16 +131,16 @@ __global__ void __launch_bounds__(256) k_residual(Box b, Tables t, ResArgs a)
             const double avVT = (a.VT[o] + a.VT[o + b.sj]) * 0.5;
             const double divV = (((dVRr + dVTt) + dVPp) + (2.0 * avVR) / R) + (avVT / R) * t.cotc[j];
             if (a.gP) a.gP[o] = (float)divV;
-            m[0] = abits(divV);
-            m[7] = abits(a.rho[o] * a.g);
-            m[8] = abits(a.tRR[o]);
-            m[9] = abits(a.tTT[o]);
-            m[10] = abits(a.tPP[o]);
-            m[11] = abits(a.P[o]);
+            m[0] = umax(m[0], abits(divV));
+            m[7] = umax(m[7], abits(a.rho[o] * a.g));
+            m[8] = umax(m[8], abits(a.tRR[o]));
+            m[9] = umax(m[9], abits(a.tTT[o]));
+            m[10] = umax(m[10], abits(a.tPP[o]));
+            m[11] = umax(m[11], abits(a.P[o]));
         }
-        if (j < nt && k < np) m[4] = abits(a.VR[o]);
-        if (i < nr && k < np) m[5] = abits(a.VT[o]);
-        if (i < nr && j < nt) m[6] = abits(a.VP[o]);
+        if (j < nt && k < np) m[4] = umax(m[4], abits(a.VR[o]));
+        if (i < nr && k < np) m[5] = umax(m[5], abits(a.VT[o]));
+        if (i < nr && j < nt) m[6] = umax(m[6], abits(a.VP[o]));
 #define SRR(q) (-a.P[q] + a.tRR[q])
 #define STT(q) (-a.P[q] + a.tTT[q])
 #define SPP(q) (-a.P[q] + a.tPP[q])
@@ -149,7 +155,7 @@ __global__ void __launch_bounds__(256) k_residual(Box b, Tables t, ResArgs a)
                                   (t.rs_f[i * (nt + 1) + j] * (a.tRP[o + 1] - a.tRP[o])) * a._dph) +
                                  (2.0 * sRR) / Rx) - (sTT + sPP) / Rx) + (aRT / Rx) * t.cotc2[j]) - rg;
             fr = (float)dV;
-            m[1] = abits(dV);
+            m[1] = umax(m[1], abits(dV));
         }
         if (i >= 1 && i <= nr - 2 && j >= 1 && j <= nt - 1 && k >= 1 && k <= np - 2) {
             const long long q = o - b.sj;
@@ -160,7 +166,7 @@ __global__ void __launch_bounds__(256) k_residual(Box b, Tables t, ResArgs a)
                                 (t.rs_y[i * (nt + 1) + j] * (a.tTP[o + 1] - a.tTP[o])) * a._dph) + (3.0 * aRT) / Ry) +
                               ((sTT - sPP) / Ry) * t.cotf2[j];
             ft = (float)dV;
-            m[2] = abits(dV);
+            m[2] = umax(m[2], abits(dV));
         }
         if (i >= 1 && i <= nr - 2 && j >= 1 && j <= nt - 2 && k >= 1 && k <= np - 1) {
             const long long q = o - 1;
@@ -170,7 +176,7 @@ __global__ void __launch_bounds__(256) k_residual(Box b, Tables t, ResArgs a)
                                 (t.rs_z[i * (nt + 1) + j] * (SPP(o) - SPP(q))) * a._dph) + (3.0 * aRP) / Rz) +
                               ((2.0 * aTP) / Rz) * t.cotc2[j];
             fp = (float)dV;
-            m[3] = abits(dV);
+            m[3] = umax(m[3], abits(dV));
         }
 #undef SRR
 #undef STT
@@ -236,7 +242,8 @@ void launch_residual(ss_ctx *c, const FieldSet &s, double *const tau[6], float *
               _dr, _dth, _dph, g, c->red};
     const Box &b = c->box;
     hipLaunchKernelGGL(k_zero_slots, dim3(1), dim3(64), 0, c->stream, c->red, 16);
-    hipLaunchKernelGGL(k_residual, mgrid(b.nr + 1, b.nt + 1, b.np + 1), MB, 0, c->stream, b, c->tab, a);
+    hipLaunchKernelGGL(k_residual, mgrid((b.nr + 1 + RES_PLANES - 1) / RES_PLANES, b.nt + 1, b.np + 1), MB, 0, c->stream,
+                       b, c->tab, a);
 }
 
 template <typename R> void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_main1);
@@ -434,6 +441,7 @@ int ss_pt_solve_mixed(ss_ctx *c, const ss_params *p, const ss_mixed_opts *o, ss_
             hipLaunchKernelGGL(k_zero_f32, dim3(2048), dim3(256), 0, c->stream, F[q], c->box_elems);
         int cur = 0, it = 0;
         double err_in = NAN;
+        const clk::time_point ti0 = clk::now(); // eval_residual ended with a synchronisation
         for (it = 1; it <= o->max_inner; ++it) {
             float *const *in = F + 10 * cur, *const *out = F + 10 * (1 - cur);
             a.VR = in[0]; a.VT = in[1]; a.VP = in[2]; a.P = in[3];
@@ -458,6 +466,8 @@ int ss_pt_solve_mixed(ss_ctx *c, const ss_params *p, const ss_mixed_opts *o, ss_
                 if (err_in < o->tol_inner) break;
             }
         }
+        if ((rc = msync(c, "inner sweeps")) != SS_OK) return rc;
+        res->inner_seconds += std::chrono::duration<double>(clk::now() - ti0).count();
         res->inner_hist[outer] = std::min(it, o->max_inner);
         res->inner_total += res->inner_hist[outer];
         res->err_inner_hist[outer] = err_in;

@@ -223,7 +223,8 @@ class Solver:
         n = r.outer
         return {"outer": n, "inner_total": r.inner_total, "converged": bool(r.converged), "r": r.r,
                 "r_hist": list(r.r_hist[:n + 1]), "inner_hist": list(r.inner_hist[:n]),
-                "err_inner_hist": list(r.err_inner_hist[:n]), "seconds": r.seconds}
+                "err_inner_hist": list(r.err_inner_hist[:n]), "seconds": r.seconds,
+                "inner_seconds": r.inner_seconds}
 
     def pt_iterate_timed(self, p: Params, niter: int):
         """pt_iterate with HIP events around each main-kernel launch; returns
