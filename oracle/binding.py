@@ -35,6 +35,15 @@ class orc_params(C.Structure):
                 ("err_drop_nan_vph", C.c_int)]
 
 
+# adjoint state, order of the ORC_A_* enum (oracle/sspt_adjoint.h)
+ADJ_FIELDS = ["VR", "Vth", "Vph", "P", "tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph",
+              "eta", "eta_PL"]
+
+
+class orc_adj(C.Structure):
+    _fields_ = [("a", DP * len(ADJ_FIELDS))]
+
+
 _LIB = None
 
 
@@ -63,6 +72,9 @@ def lib():
             "orc_pt_step": (None, [SP, PP, I]),
             "orc_pt_solve": (I, [SP, PP, I, I, D, I, DP, C.POINTER(I)]),
             "orc_num_threads": (I, []),
+            "orc_adj_step": (None, [SP, PP, C.POINTER(orc_adj), DP, DP]),
+            "orc_adj_objective": (D, [SP, D, D, DP]),
+            "orc_adj_solve": (I, [SP, PP, D, D, C.POINTER(orc_adj), I, I, D, DP, DP]),
         }
         for n, (res, args) in sig.items():
             f = getattr(L, n)
@@ -161,6 +173,42 @@ class Oracle:
         ittot = self.L.orc_pt_solve(C.byref(self.s), C.byref(self.p), int(maxiter), int(nout), float(eps),
                                     int(always_equal3), C.byref(err), C.byref(it_end))
         return {"ittot": ittot, "iter_end": it_end.value, "err": err.value}
+
+
+    # ---- adjoint of the PT iteration at a fixed point (oracle/sspt_adjoint.c)
+    def adj_zero(self):
+        return {n: np.zeros(self.f[n].shape, order="F") for n in ADJ_FIELDS}
+
+    def _adj_struct(self, lam):
+        arrs = {n: np.array(lam[n], dtype=np.float64, order="F", copy=True) for n in ADJ_FIELDS}
+        a = orc_adj()
+        for q, n in enumerate(ADJ_FIELDS):
+            a.a[q] = _ptr(arrs[n])
+        return a, arrs
+
+    def adj_step(self, lam, grads=False):
+        """lam -> T^T lam (a new dict); with grads also (dPhi/drho)^T lam and
+        (dPhi/deta_ini)^T lam."""
+        a, arrs = self._adj_struct(lam)
+        g_rho = np.zeros(self.f["rho"].shape, order="F") if grads else None
+        g_ini = np.zeros(self.f["eta_ini"].shape, order="F") if grads else None
+        self.L.orc_adj_step(C.byref(self.s), C.byref(self.p), C.byref(a),
+                            _ptr(g_rho) if grads else None, _ptr(g_ini) if grads else None)
+        return (arrs, g_rho, g_ini) if grads else arrs
+
+    def objective(self, r, radius):
+        dJ = np.zeros(self.f["VR"].shape, order="F")
+        J = self.L.orc_adj_objective(C.byref(self.s), float(r), float(radius), _ptr(dJ))
+        return J, dJ
+
+    def adj_solve(self, r, radius, maxiter, ncheck, tol):
+        """Adjoint of the converged solve: (iterations, lam, dJ/drho, dJ/deta_ini)."""
+        a, arrs = self._adj_struct(self.adj_zero())
+        g_rho = np.zeros(self.f["rho"].shape, order="F")
+        g_ini = np.zeros(self.f["eta_ini"].shape, order="F")
+        it = self.L.orc_adj_solve(C.byref(self.s), C.byref(self.p), float(r), float(radius), C.byref(a),
+                                  int(maxiter), int(ncheck), float(tol), _ptr(g_rho), _ptr(g_ini))
+        return it, arrs, g_rho, g_ini
 
 
 def num_threads() -> int:
