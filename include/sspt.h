@@ -175,6 +175,46 @@ typedef struct ss_mixed_result {
  * Converges to the FP64 solution of the same equations. */
 int ss_pt_solve_mixed(ss_ctx *ctx, const ss_params *p, const ss_mixed_opts *o, ss_mixed_result *res);
 
+/* ---- adjoint of the converged PT solve (no reference counterpart) ----
+ * At a converged state x* = Phi(x*) of the PT iteration Phi, the gradient of
+ * J = the volume-weighted mean VR over the r-faces inside the inclusion
+ * (face centre (R_f, th, ph) with (R_f - r)^2 + ((th - pi/2) r)^2 + (ph r)^2 <
+ * radius, the squared-distance test of initialize_inclusion!, weight
+ * R_f^2 sin(th)) with respect to the per-cell rho and eta_ini is
+ * (dPhi/dm)^T lam with lam = T^T lam + dJ/dx, T = dPhi/dx at x* (step sizes
+ * held at x*, exact at the fixed point).  lam is iterated with the transposed
+ * PT map (the forward iteration's contraction); only x* is used, no record
+ * of the forward iterations.  Call after ss_pt_iterate/ss_pt_solve (their
+ * last iteration leaves etaSM, DT_R and tII of the final state). */
+enum ss_adj_field {
+    SS_ADJ_VR = 0, SS_ADJ_VTH, SS_ADJ_VPH, SS_ADJ_P, SS_ADJ_TAU_RR, SS_ADJ_TAU_THTH, SS_ADJ_TAU_PHPH,
+    SS_ADJ_TAU_RTH, SS_ADJ_TAU_RPH, SS_ADJ_TAU_THPH, SS_ADJ_ETA, SS_ADJ_ETA_PL,
+    SS_ADJ_GRAD_RHO,      /* dJ/drho (centre shape) */
+    SS_ADJ_GRAD_ETA_INI,  /* dJ/deta_ini (centre shape); d/dlog10(eta_ini) = this * eta_ini * ln 10 */
+    SS_ADJ_NFIELDS
+};
+typedef struct ss_adjoint_result {
+    int iters, converged;
+    double J;        /* objective at the state */
+    double err;      /* last max|dlam|/max|lam| over the velocity components */
+    double seconds;
+} ss_adjoint_result;
+/* Coefficients of T at the current state (allocates the adjoint boxes on first use). */
+int ss_adjoint_prepare(ss_ctx *ctx, const ss_params *p);
+/* lam = 0 and gradients = 0. */
+int ss_adjoint_zero(ss_ctx *ctx);
+/* J at the current state; keeps dJ/dVR for ss_adjoint_solve. */
+int ss_adjoint_objective(ss_ctx *ctx, double r, double radius, double *J);
+int ss_adjoint_upload(ss_ctx *ctx, int adj_field, const double *host, int layout);
+int ss_adjoint_download(ss_ctx *ctx, int adj_field, double *host, int layout);
+/* lam <- T^T lam; with gradients != 0 also grad += (dPhi/dm)^T lam (before the step). */
+int ss_adjoint_step(ss_ctx *ctx, const ss_params *p, int gradients);
+/* prepare + objective + lam = T^T lam + dJ/dx until max|dlam|/max|lam| < tol
+ * (checked every ncheck iterations) + gradients.  SS_NOT_CONVERGED if maxiter
+ * was reached (the gradients are those of the last lam). */
+int ss_adjoint_solve(ss_ctx *ctx, const ss_params *p, double r, double radius, int maxiter, int ncheck, double tol,
+                     ss_adjoint_result *out);
+
 /* ---- timing hooks for the benchmark (no reference counterpart) ---- */
 /* Average device time (ms) per iteration of the last ss_pt_iterate, from HIP
  * events on the context's stream. */

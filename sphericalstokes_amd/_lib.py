@@ -53,6 +53,15 @@ class ss_mixed_result(C.Structure):
                 ("inner_seconds", C.c_double)]
 
 
+class ss_adjoint_result(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("J", C.c_double), ("err", C.c_double),
+                ("seconds", C.c_double)]
+
+
+# adjoint field ids (enum ss_adj_field)
+ADJ_FIELDS = ["VR", "Vth", "Vph", "P", "tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph", "eta",
+              "eta_PL", "grad_rho", "grad_eta_ini"]
+
 # name -> (restype, argtypes); every symbol declared in include/sspt.h
 P = C.c_void_p
 D = C.c_double
@@ -88,6 +97,13 @@ SIGNATURES = {
     "ss_residual": (I, [P, C.POINTER(ss_params), DP]),
     "ss_pt_solve_residual": (I, [P, C.POINTER(ss_params), I, I, D, C.POINTER(ss_result)]),
     "ss_pt_solve_mixed": (I, [P, C.POINTER(ss_params), C.POINTER(ss_mixed_opts), C.POINTER(ss_mixed_result)]),
+    "ss_adjoint_prepare": (I, [P, C.POINTER(ss_params)]),
+    "ss_adjoint_zero": (I, [P]),
+    "ss_adjoint_objective": (I, [P, D, D, DP]),
+    "ss_adjoint_upload": (I, [P, I, DP, I]),
+    "ss_adjoint_download": (I, [P, I, DP, I]),
+    "ss_adjoint_step": (I, [P, C.POINTER(ss_params), I]),
+    "ss_adjoint_solve": (I, [P, C.POINTER(ss_params), D, D, I, I, D, C.POINTER(ss_adjoint_result)]),
     "ss_last_kernel_ms": (I, [P, DP]),
     "ss_pt_iterate_timed": (I, [P, C.POINTER(ss_params), I, DP, DP]),
 }

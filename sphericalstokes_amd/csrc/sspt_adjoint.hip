@@ -1,0 +1,923 @@
+// sspt_adjoint.hip -- adjoint of the PT iteration at a converged state.
+//
+// With x* = Phi(x*) the converged state of one PT iteration Phi (the
+// compute_P! / compute_TII! + copy_BC! / power_law! / compute_V! / BC
+// sequence, SphericalStokes_diapir.jl:413-425) and J(x) = the
+// volume-weighted mean VR over the r-faces inside the inclusion,
+//     dJ/dm = (dPhi/dm)^T lam,   lam = T^T lam + dJ/dx,   T = dPhi/dx (x*),
+// for m = rho and eta_ini.  T is taken with the step sizes of x* fixed
+// (etaSM through KBDT, GSDT and DT_R): they multiply residuals that vanish
+// at x*, so this is the exact Jacobian there.  The iteration for lam is the
+// transpose of the forward PT map: same spectrum, same convergence rate, and
+// it needs only the converged state, no record of the forward iterations.
+//
+// T^T is applied in gather form, one kernel per family of outputs, with no
+// atomics (each output sums the coefficients of the forward statements that
+// read it); the shifted storage of the shear stresses makes every transposed
+// stencil a +-1 offset in the box frame:
+//   1. BC fold: ghost adjoints go to the points their values came from
+//      (BC_ph!, BC_th!, BC_R! backwards);
+//   2. power law: lam_tII of every cell from lam_eta (and lam_eta_PL);
+//   3. interior cells: Q = (sum of lam_tII over the cells that copy_BC!
+//      filled from them) / tII;
+//   4. centres and edges: adjoints of the new P and tau (compute_V!'s and
+//      compute_TII!'s transposes), written scaled by compute_P!'s 1/den;
+//   5. faces and centres: lam_V (compute_P!'s strain-rate transposes) and
+//      lam_eta (its viscosity dependence).
+#include "sspt_internal.h"
+#include "sspt_launch.h"
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace sspt {
+
+// adjoint boxes held by the context
+enum {
+    AD_SETA = 0,   // lam set A: VR VT VP P RR TT PP RT RP TP E PL (12)
+    AD_SETB = 12,  // lam set B
+    AD_PREV = 24,  // VR VT VP of the previous iterate (convergence check)
+    AD_LT = 27, AD_Q = 28, AD_BJ = 29, AD_GRHO = 30, AD_GINI = 31,
+    AD_SID = 32, AD_SGS = 33, AD_SKB = 34, AD_SAE = 35, AD_SX = 36, AD_SY = 37, AD_SZ = 38, AD_SIT = 39,
+    AD_GRT = 40, AD_IDRT = 41, AD_CRT = 42, AD_GRP = 43, AD_IDRP = 44, AD_CRP = 45,
+    AD_GTP = 46, AD_IDTP = 47, AD_CTP = 48,
+    AD_NBOX = 49
+};
+static_assert(AD_NBOX == SS_ADJ_NBOX, "adjoint box count");
+enum { L_VR, L_VT, L_VP, L_P, L_RR, L_TT, L_PP, L_RT, L_RP, L_TP, L_E, L_PL, L_N };
+
+struct AdjSet {
+    double *f[L_N];
+};
+
+struct AdjArgs {
+    Box b;
+    Tables t;
+    AdjSet in, out;
+    // converged forward state
+    const double *tRR, *tTT, *tPP, *tRT, *tRP, *tTP, *eta, *ini, *ePL, *tII, *DT, *SM, *VR;
+    // fixed coefficients of the linearisation
+    double *sID, *sGS, *sKB, *sAE, *sX, *sY, *sZ, *sIT;
+    double *gRT, *idRT, *cRT, *gRP, *idRP, *cRP, *gTP, *idTP, *cTP;
+    double *LT, *Q, *bJ, *gRho, *gIni;
+    double _dr, _dth, _dph, g, kb, gs, tC, ex, relax, emin, emax;
+    int do_relax, do_lo, do_hi, grad, bc_mode;
+};
+
+#define A_LOOP(NI, NJ, NK)                                        \
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;          \
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;          \
+    const int i = blockIdx.z;                                     \
+    if (i >= (NI) || j >= (NJ) || k >= (NK)) return;
+
+static const dim3 AB(64, 4, 1);
+static inline dim3 agrid(int ni, int nj, int nk) { return dim3((nk + 63) / 64, (nj + 3) / 4, ni); }
+
+// ------------------------------------------------------------------ ranges
+// updated faces of compute_V! and the edge boxes, in box indices
+__device__ __forceinline__ bool upd_VR(const Box &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 1 && j >= 1 && j <= b.nt - 2 && k >= 1 && k <= b.np - 2;
+}
+__device__ __forceinline__ bool upd_VT(const Box &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 2 && j >= 1 && j <= b.nt - 1 && k >= 1 && k <= b.np - 2;
+}
+__device__ __forceinline__ bool upd_VP(const Box &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 2 && j >= 1 && j <= b.nt - 2 && k >= 1 && k <= b.np - 1;
+}
+__device__ __forceinline__ bool in_RT(const Box &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 1 && j >= 1 && j <= b.nt - 1 && k >= 1 && k <= b.np - 2;
+}
+__device__ __forceinline__ bool in_RP(const Box &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 1 && j >= 1 && j <= b.nt - 2 && k >= 1 && k <= b.np - 1;
+}
+__device__ __forceinline__ bool in_TP(const Box &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 2 && j >= 1 && j <= b.nt - 1 && k >= 1 && k <= b.np - 1;
+}
+__device__ __forceinline__ bool inner_c(const Box &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 2 && j >= 1 && j <= b.nt - 2 && k >= 1 && k <= b.np - 2;
+}
+
+// w_f = DT_f lam_V_f of the faces compute_V! updates (DT_f: average of the
+// two centres), 0 elsewhere
+__device__ __forceinline__ double w_VR(const AdjArgs &a, int i, int j, int k)
+{
+    if (!upd_VR(a.b, i, j, k)) return 0.0;
+    const long long o = a.b.off(i, j, k);
+    return ((a.DT[o - a.b.si] + a.DT[o]) * 0.5) * a.in.f[L_VR][o];
+}
+__device__ __forceinline__ double w_VT(const AdjArgs &a, int i, int j, int k)
+{
+    if (!upd_VT(a.b, i, j, k)) return 0.0;
+    const long long o = a.b.off(i, j, k);
+    return ((a.DT[o - a.b.sj] + a.DT[o]) * 0.5) * a.in.f[L_VT][o];
+}
+__device__ __forceinline__ double w_VP(const AdjArgs &a, int i, int j, int k)
+{
+    if (!upd_VP(a.b, i, j, k)) return 0.0;
+    const long long o = a.b.off(i, j, k);
+    return ((a.DT[o - 1] + a.DT[o]) * 0.5) * a.in.f[L_VP][o];
+}
+
+// ------------------------------------------------------------------ fixed coefficients
+__global__ void __launch_bounds__(256) k_adj_prep_centre(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    const Box &b = a.b;
+    const long long o = b.off(i, j, k);
+    const double GS = a.gs * a.SM[o], e = a.eta[o];
+    const double den = 1.0 + GS / e;
+    a.sGS[o] = GS;
+    a.sKB[o] = a.kb * a.SM[o];
+    a.sID[o] = 1.0 / den;
+    a.sAE[o] = GS / (e * e);
+    double it = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    if (inner_c(b, i, j, k) && a.tII[o] > 0.0) {
+        it = 1.0 / a.tII[o];
+        x = (((a.tRT[o] + a.tRT[o + b.sj]) + a.tRT[o + b.si]) + a.tRT[o + b.si + b.sj]) * 0.25;
+        y = (((a.tRP[o] + a.tRP[o + 1]) + a.tRP[o + b.si]) + a.tRP[o + b.si + 1]) * 0.25;
+        z = (((a.tTP[o] + a.tTP[o + 1]) + a.tTP[o + b.sj]) + a.tTP[o + b.sj + 1]) * 0.25;
+    }
+    a.sIT[o] = it;
+    a.sX[o] = 0.25 * x;
+    a.sY[o] = 0.25 * y;
+    a.sZ[o] = 0.25 * z;
+}
+
+// edge families: 0 = tau_Rth, 1 = tau_Rph, 2 = tau_thph
+__global__ void __launch_bounds__(256) k_adj_prep_edge(AdjArgs a, int fam)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    const Box &b = a.b;
+    const long long o = b.off(i, j, k);
+    long long c00, c01, c10;
+    const double *tau;
+    double *G, *ID, *CC;
+    if (fam == 0) {
+        if (!in_RT(b, i, j, k)) return;
+        c00 = o - b.si - b.sj; c01 = o - b.si; c10 = o - b.sj; tau = a.tRT; G = a.gRT; ID = a.idRT; CC = a.cRT;
+    } else if (fam == 1) {
+        if (!in_RP(b, i, j, k)) return;
+        c00 = o - b.si - 1; c01 = o - b.si; c10 = o - 1; tau = a.tRP; G = a.gRP; ID = a.idRP; CC = a.cRP;
+    } else {
+        if (!in_TP(b, i, j, k)) return;
+        c00 = o - b.sj - 1; c01 = o - b.sj; c10 = o - 1; tau = a.tTP; G = a.gTP; ID = a.idTP; CC = a.cTP;
+    }
+    const double g4 = (((a.gs * a.SM[c00] + a.gs * a.SM[c01]) + a.gs * a.SM[c10]) + a.gs * a.SM[o]) * 0.25;
+    const double e4 = (((a.eta[c00] + a.eta[c01]) + a.eta[c10]) + a.eta[o]) * 0.25;
+    G[o] = g4;
+    ID[o] = 1.0 / (1.0 + g4 / e4);
+    CC[o] = 0.25 * tau[o] * (g4 / (e4 * e4));
+}
+
+// ------------------------------------------------------------------ 1. BC fold
+// Threads cover the two other directions; each thread handles both sides of
+// its line (with 3 points in a direction both sides fold onto the same point).
+__global__ void k_adj_bc_ph(AdjArgs a)
+{
+    const Box &b = a.b;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    const bool cp = a.bc_mode != SS_BC_SHEAR;
+    double *VR = a.in.f[L_VR], *VT = a.in.f[L_VT], *VP = a.in.f[L_VP];
+    if (i < nr && j < nt) { // Vph: ghost = c - inner
+        const long long lo = b.off(i, j, 0), hi = b.off(i, j, np);
+        VP[lo + 1] -= VP[lo]; VP[lo] = 0.0;
+        VP[hi - 1] -= VP[hi]; VP[hi] = 0.0;
+    }
+    if (i < nr && j <= nt) {
+        const long long lo = b.off(i, j, 0), hi = b.off(i, j, np - 1);
+        if (cp) { VT[lo + 1] += VT[lo]; VT[hi - 1] += VT[hi]; }
+        VT[lo] = 0.0; VT[hi] = 0.0;
+    }
+    if (i <= nr && j < nt) {
+        const long long lo = b.off(i, j, 0), hi = b.off(i, j, np - 1);
+        if (cp) { VR[lo + 1] += VR[lo]; VR[hi - 1] += VR[hi]; }
+        VR[lo] = 0.0; VR[hi] = 0.0;
+    }
+}
+
+__global__ void k_adj_bc_th(AdjArgs a)
+{
+    const Box &b = a.b;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    const bool cp = a.bc_mode != SS_BC_SHEAR;
+    const long long sj = b.sj;
+    double *VR = a.in.f[L_VR], *VT = a.in.f[L_VT], *VP = a.in.f[L_VP];
+    if (i < nr && k <= np) {
+        const long long lo = b.off(i, 0, k), hi = b.off(i, nt - 1, k);
+        if (cp) { VP[lo + sj] += VP[lo]; VP[hi - sj] += VP[hi]; }
+        VP[lo] = 0.0; VP[hi] = 0.0;
+    }
+    if (i < nr && k < np) { // Vth: ghost = c - inner
+        const long long lo = b.off(i, 0, k), hi = b.off(i, nt, k);
+        VT[lo + sj] -= VT[lo]; VT[lo] = 0.0;
+        VT[hi - sj] -= VT[hi]; VT[hi] = 0.0;
+    }
+    if (i <= nr && k < np) {
+        const long long lo = b.off(i, 0, k), hi = b.off(i, nt - 1, k);
+        if (cp) { VR[lo + sj] += VR[lo]; VR[hi - sj] += VR[hi]; }
+        VR[lo] = 0.0; VR[hi] = 0.0;
+    }
+}
+
+__global__ void k_adj_bc_R(AdjArgs a)
+{
+    const Box &b = a.b;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    const long long si = b.si;
+    double *VR = a.in.f[L_VR], *VT = a.in.f[L_VT], *VP = a.in.f[L_VP];
+    if (j < nt && k <= np) {
+        const long long lo = b.off(0, j, k), hi = b.off(nr - 1, j, k);
+        VP[lo + si] += VP[lo]; VP[lo] = 0.0;
+        VP[hi - si] += VP[hi]; VP[hi] = 0.0;
+    }
+    if (j <= nt && k < np) {
+        const long long lo = b.off(0, j, k), hi = b.off(nr - 1, j, k);
+        VT[lo + si] += VT[lo]; VT[lo] = 0.0;
+        VT[hi - si] += VT[hi]; VT[hi] = 0.0;
+    }
+    if (j < nt && k < np) { // VR: ghost = -inner
+        const long long lo = b.off(0, j, k), hi = b.off(nr, j, k);
+        VR[lo + si] -= VR[lo]; VR[lo] = 0.0;
+        VR[hi - si] -= VR[hi]; VR[hi] = 0.0;
+    }
+}
+
+// ------------------------------------------------------------------ 2. power law
+__global__ void __launch_bounds__(256) k_adj_power_law(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    const long long o = a.b.off(i, j, k);
+    const double le = a.in.f[L_E][o];
+    const double tii = a.tII[o], ini = a.ini[o];
+    const double pl0 = ini * pow(tii / a.tC, a.ex);
+    const double pl = a.do_relax ? a.ePL[o] : pl0;
+    const double e_raw = 2.0 / (1.0 / ini + 1.0 / pl);
+    const bool clamped = (a.do_lo && e_raw < a.emin) || (a.do_hi && e_raw > a.emax);
+    const double a_e = clamped ? 0.0 : le;
+    if (!isfinite(pl) || !isfinite(pl0)) { // tII = 0: eta = 2 eta_ini
+        a.LT[o] = 0.0;
+        if (a.grad) a.gIni[o] += a_e * 2.0;
+        a.out.f[L_PL][o] = 0.0;
+        return;
+    }
+    double l_pl = a_e * (e_raw * e_raw) / (2.0 * pl * pl);
+    double l_ini = a_e * (e_raw * e_raw) / (2.0 * ini * ini);
+    if (a.do_relax) {
+        l_pl += a.in.f[L_PL][o];
+        const double l_pl0 = l_pl * a.relax * pl / pl0;
+        a.out.f[L_PL][o] = l_pl * (1.0 - a.relax) * pl / a.ePL[o];
+        l_pl = l_pl0;
+    } else {
+        a.out.f[L_PL][o] = 0.0;
+    }
+    a.LT[o] = (tii > 0.0) ? l_pl * a.ex * pl0 / tii : 0.0;
+    l_ini += l_pl * pl0 / ini;
+    if (a.grad) a.gIni[o] += l_ini;
+}
+
+// ------------------------------------------------------------------ 3. tII of interior cells
+__global__ void __launch_bounds__(256) k_adj_tii(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    const Box &b = a.b;
+    const long long o = b.off(i, j, k);
+    if (!inner_c(b, i, j, k)) { a.Q[o] = 0.0; return; }
+    // copy_BC! filled a boundary cell from the interior cell at its clamped
+    // position: this cell collects its own lam_tII and those of the boundary
+    // cells that clamp onto it
+    int di[3] = {0, 0, 0}, dj[3] = {0, 0, 0}, dk[3] = {0, 0, 0};
+    int ni = 1, nj = 1, nk = 1;
+    if (i == 1) di[ni++] = -1;
+    if (i == b.nr - 2) di[ni++] = 1;
+    if (j == 1) dj[nj++] = -1;
+    if (j == b.nt - 2) dj[nj++] = 1;
+    if (k == 1) dk[nk++] = -1;
+    if (k == b.np - 2) dk[nk++] = 1;
+    double s = 0.0;
+    for (int p = 0; p < ni; ++p)
+        for (int q = 0; q < nj; ++q)
+            for (int r = 0; r < nk; ++r) s += a.LT[o + di[p] * b.si + dj[q] * b.sj + dk[r]];
+    a.Q[o] = s * a.sIT[o];
+}
+
+// ------------------------------------------------------------------ 4. new-stress adjoints
+// centres: P, tau_nn
+__global__ void __launch_bounds__(256) k_adj_sig_centre(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    const Box &b = a.b;
+    const Tables &t = a.t;
+    const int nt = b.nt;
+    const long long o = b.off(i, j, k);
+    const double wR0 = w_VR(a, i, j, k), wR1 = w_VR(a, i + 1, j, k);
+    const double wT0 = w_VT(a, i, j, k), wT1 = w_VT(a, i, j + 1, k);
+    const double wP0 = w_VP(a, i, j, k), wP1 = w_VP(a, i, j, k + 1);
+    double sRR = 0.0, sTT = 0.0, sPP = 0.0;
+    if (wR0 != 0.0) {
+        const double iR = 1.0 / t.Rf[i];
+        sRR += wR0 * (a._dr + iR);
+        sTT -= 0.5 * wR0 * iR;
+        sPP -= 0.5 * wR0 * iR;
+    }
+    if (wR1 != 0.0) {
+        const double iR = 1.0 / t.Rf[i + 1];
+        sRR += wR1 * (-a._dr + iR);
+        sTT -= 0.5 * wR1 * iR;
+        sPP -= 0.5 * wR1 * iR;
+    }
+    const double iRy = 1.0 / t.Rc2[i];
+    if (wT0 != 0.0) {
+        const double ct = t.cotf2[j];
+        sTT += wT0 * (a._dth * iRy + 0.5 * ct * iRy);
+        sPP -= 0.5 * wT0 * ct * iRy;
+    }
+    if (wT1 != 0.0) {
+        const double ct = t.cotf2[j + 1];
+        sTT += wT1 * (-a._dth * iRy + 0.5 * ct * iRy);
+        sPP -= 0.5 * wT1 * ct * iRy;
+    }
+    const double rz = t.rs_z[i * (nt + 1) + j] * a._dph;
+    sPP += (wP0 - wP1) * rz;
+    double nRR = a.in.f[L_RR][o] + sRR, nTT = a.in.f[L_TT][o] + sTT, nPP = a.in.f[L_PP][o] + sPP;
+    if (inner_c(b, i, j, k)) {
+        const double q = 0.5 * a.Q[o];
+        nRR += q * a.tRR[o];
+        nTT += q * a.tTT[o];
+        nPP += q * a.tPP[o];
+    }
+    a.out.f[L_P][o] = a.in.f[L_P][o] - ((sRR + sTT) + sPP);
+    const double id = a.sID[o];
+    a.out.f[L_RR][o] = nRR * id;
+    a.out.f[L_TT][o] = nTT * id;
+    a.out.f[L_PP][o] = nPP * id;
+}
+
+__global__ void __launch_bounds__(256) k_adj_sig_edge(AdjArgs a, int fam)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    const Box &b = a.b;
+    const Tables &t = a.t;
+    const int nt = b.nt;
+    const long long o = b.off(i, j, k);
+    double s = 0.0, tq = 0.0;
+    if (fam == 0) {
+        if (!in_RT(b, i, j, k)) return;
+        const double iR = 1.0 / t.Rf[i];
+        s += w_VR(a, i, j - 1, k) * (a._dth * iR + 0.5 * t.cotc2[j - 1] * iR);
+        s += w_VR(a, i, j, k) * (-a._dth * iR + 0.5 * t.cotc2[j] * iR);
+        s += w_VT(a, i - 1, j, k) * (a._dr + 1.5 / t.Rc2[i - 1]);
+        s += w_VT(a, i, j, k) * (-a._dr + 1.5 / t.Rc2[i]);
+        for (int di = 0; di <= 1; ++di)
+            for (int dj = 0; dj <= 1; ++dj)
+                if (inner_c(b, i - di, j - dj, k)) {
+                    const long long c = o - di * b.si - dj * b.sj;
+                    tq += a.Q[c] * a.sX[c];
+                }
+        a.out.f[L_RT][o] = ((a.in.f[L_RT][o] + s) + tq) * a.idRT[o];
+    } else if (fam == 1) {
+        if (!in_RP(b, i, j, k)) return;
+        const double rf = t.rs_f[i * (nt + 1) + j] * a._dph;
+        s += (w_VR(a, i, j, k - 1) - w_VR(a, i, j, k)) * rf;
+        s += w_VP(a, i - 1, j, k) * (a._dr + 1.5 / t.Rc2[i - 1]);
+        s += w_VP(a, i, j, k) * (-a._dr + 1.5 / t.Rc2[i]);
+        for (int di = 0; di <= 1; ++di)
+            for (int dk = 0; dk <= 1; ++dk)
+                if (inner_c(b, i - di, j, k - dk)) {
+                    const long long c = o - di * b.si - dk;
+                    tq += a.Q[c] * a.sY[c];
+                }
+        a.out.f[L_RP][o] = ((a.in.f[L_RP][o] + s) + tq) * a.idRP[o];
+    } else {
+        if (!in_TP(b, i, j, k)) return;
+        const double ry = t.rs_y[i * (nt + 1) + j] * a._dph;
+        s += (w_VT(a, i, j, k - 1) - w_VT(a, i, j, k)) * ry;
+        const double iR = 1.0 / t.Rc2[i];
+        s += w_VP(a, i, j - 1, k) * (a._dth * iR + t.cotc2[j - 1] * iR);
+        s += w_VP(a, i, j, k) * (-a._dth * iR + t.cotc2[j] * iR);
+        for (int dj = 0; dj <= 1; ++dj)
+            for (int dk = 0; dk <= 1; ++dk)
+                if (inner_c(b, i, j - dj, k - dk)) {
+                    const long long c = o - dj * b.sj - dk;
+                    tq += a.Q[c] * a.sZ[c];
+                }
+        a.out.f[L_TP][o] = ((a.in.f[L_TP][o] + s) + tq) * a.idTP[o];
+    }
+}
+
+// ------------------------------------------------------------------ 5. lam_V and lam_eta
+// Weights of compute_P!'s centre stencil from the new-stress adjoints of
+// cell c: (w_dVRr, w_dVTt, w_dVPp, w_avVR, w_avVT).  The out set holds
+// lam_tnn' / den, and (2 GS / den) lam_tnn' = 2 GS lam_out.
+struct CW {
+    double dR, dT, dP, aR, aT;
+};
+__device__ __forceinline__ CW centre_w(const AdjArgs &a, long long c, int i, int j)
+{
+    const double GS2 = 2.0 * a.sGS[c];
+    const double wR = GS2 * a.out.f[L_RR][c], wT = GS2 * a.out.f[L_TT][c], wP = GS2 * a.out.f[L_PP][c];
+    const double wdiv = -a.sKB[c] * a.out.f[L_P][c] - (wR + wT + wP) * (1.0 / 3.0);
+    const double iR = a.t.invRc[i];
+    CW w;
+    w.dR = wR + wdiv;
+    w.dT = wT + wdiv;
+    w.dP = wP + wdiv;
+    w.aR = (wT + wP) * iR + 2.0 * wdiv * iR;
+    w.aT = (wP + wdiv) * a.t.cotc[j] * iR;
+    return w;
+}
+
+__global__ void __launch_bounds__(256) k_adj_vr(AdjArgs a)
+{
+    A_LOOP(a.b.nr + 1, a.b.nt, a.b.np)
+    const Box &b = a.b;
+    const Tables &t = a.t;
+    const int nt = b.nt;
+    const long long o = b.off(i, j, k);
+    double v = a.in.f[L_VR][o];
+    if (i >= 1) { // upper face of cell i-1
+        const CW w = centre_w(a, o - b.si, i - 1, j);
+        v += w.dR * a._dr + 0.5 * w.aR;
+    }
+    if (i <= b.nr - 1) { // lower face of cell i
+        const CW w = centre_w(a, o, i, j);
+        v += -w.dR * a._dr + 0.5 * w.aR;
+    }
+    if (in_RT(b, i, j, k)) v += a.gRT[o] * a.out.f[L_RT][o] * a._dth / t.Re[i];
+    if (in_RT(b, i, j + 1, k)) v -= a.gRT[o + b.sj] * a.out.f[L_RT][o + b.sj] * a._dth / t.Re[i];
+    if (in_RP(b, i, j, k) || in_RP(b, i, j, k + 1)) {
+        const double re = t.rs_e[i * (nt + 1) + j] * a._dph;
+        if (in_RP(b, i, j, k)) v += a.gRP[o] * a.out.f[L_RP][o] * re;
+        if (in_RP(b, i, j, k + 1)) v -= a.gRP[o + 1] * a.out.f[L_RP][o + 1] * re;
+    }
+    if (a.bJ) v += a.bJ[o];
+    a.out.f[L_VR][o] = v;
+}
+
+__global__ void __launch_bounds__(256) k_adj_vt(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt + 1, a.b.np)
+    const Box &b = a.b;
+    const Tables &t = a.t;
+    const int nt = b.nt;
+    const long long o = b.off(i, j, k);
+    double v = a.in.f[L_VT][o];
+    const double dth_R = a._dth * t.invRc[i];
+    if (j >= 1) { // upper th-face of cell j-1
+        const CW w = centre_w(a, o - b.sj, i, j - 1);
+        v += w.dT * dth_R + 0.5 * w.aT;
+    }
+    if (j <= nt - 1) {
+        const CW w = centre_w(a, o, i, j);
+        v += -w.dT * dth_R + 0.5 * w.aT;
+    }
+    if (in_RT(b, i, j, k)) v += a.gRT[o] * a.out.f[L_RT][o] * (a._dr - 0.5 / t.Re[i]);
+    if (in_RT(b, i + 1, j, k)) v += a.gRT[o + b.si] * a.out.f[L_RT][o + b.si] * (-a._dr - 0.5 / t.Re[i + 1]);
+    if (in_TP(b, i, j, k) || in_TP(b, i, j, k + 1)) {
+        const double r4 = t.rs_c4[i * (nt + 1) + j] * a._dph;
+        if (in_TP(b, i, j, k)) v += a.gTP[o] * a.out.f[L_TP][o] * r4;
+        if (in_TP(b, i, j, k + 1)) v -= a.gTP[o + 1] * a.out.f[L_TP][o + 1] * r4;
+    }
+    a.out.f[L_VT][o] = v;
+}
+
+__global__ void __launch_bounds__(256) k_adj_vp(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np + 1)
+    const Box &b = a.b;
+    const Tables &t = a.t;
+    const int nt = b.nt;
+    const long long o = b.off(i, j, k);
+    double v = a.in.f[L_VP][o];
+    const double rc = t.rs_c[i * (nt + 1) + j] * a._dph;
+    if (k >= 1) {
+        const CW w = centre_w(a, o - 1, i, j);
+        v += w.dP * rc;
+    }
+    if (k <= b.np - 1) {
+        const CW w = centre_w(a, o, i, j);
+        v -= w.dP * rc;
+    }
+    if (in_RP(b, i, j, k)) v += a.gRP[o] * a.out.f[L_RP][o] * (a._dr - 0.5 / t.Re[i]);
+    if (in_RP(b, i + 1, j, k)) v += a.gRP[o + b.si] * a.out.f[L_RP][o + b.si] * (-a._dr - 0.5 / t.Re[i + 1]);
+    if (in_TP(b, i, j, k)) {
+        const double iR = 1.0 / t.Rc4[i];
+        v += a.gTP[o] * a.out.f[L_TP][o] * (a._dth * iR - 0.5 * t.cotf4[j] * iR);
+    }
+    if (in_TP(b, i, j + 1, k)) {
+        const double iR = 1.0 / t.Rc4[i];
+        v += a.gTP[o + b.sj] * a.out.f[L_TP][o + b.sj] * (-a._dth * iR - 0.5 * t.cotf4[j + 1] * iR);
+    }
+    a.out.f[L_VP][o] = v;
+}
+
+__global__ void __launch_bounds__(256) k_adj_eta(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    const Box &b = a.b;
+    const long long o = b.off(i, j, k);
+    double v = ((a.out.f[L_RR][o] * a.tRR[o] + a.out.f[L_TT][o] * a.tTT[o]) + a.out.f[L_PP][o] * a.tPP[o]) * a.sAE[o];
+    for (int d1 = 0; d1 <= 1; ++d1)
+        for (int d2 = 0; d2 <= 1; ++d2) {
+            if (in_RT(b, i + d1, j + d2, k)) {
+                const long long e = o + d1 * b.si + d2 * b.sj;
+                v += a.cRT[e] * a.out.f[L_RT][e];
+            }
+            if (in_RP(b, i + d1, j, k + d2)) {
+                const long long e = o + d1 * b.si + d2;
+                v += a.cRP[e] * a.out.f[L_RP][e];
+            }
+            if (in_TP(b, i, j + d1, k + d2)) {
+                const long long e = o + d1 * b.sj + d2;
+                v += a.cTP[e] * a.out.f[L_TP][e];
+            }
+        }
+    a.out.f[L_E][o] = v;
+}
+
+// gradient with respect to rho: compute_V!'s -rho g on VR faces
+__global__ void __launch_bounds__(256) k_adj_grad_rho(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    const long long o = a.b.off(i, j, k);
+    a.gRho[o] += -0.5 * a.g * (w_VR(a, i, j, k) + w_VR(a, i + 1, j, k));
+}
+
+// ------------------------------------------------------------------ objective
+// J = sum w VR / sum w over interior r-faces inside the inclusion; bJ = w
+// (scaled by 1/sum w afterwards).  Per-block partial sums, then one block.
+constexpr int OBJ_BLOCK = 256;
+__global__ void __launch_bounds__(OBJ_BLOCK) k_adj_obj_partial(AdjArgs a, double r, double radius, double *part)
+{
+    const Box &b = a.b;
+    const long long nface = (long long)(b.nr + 1) * b.nt * b.np;
+    double sw = 0.0, swv = 0.0;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nface; q += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(q % b.np), j = (int)((q / b.np) % b.nt), i = (int)(q / ((long long)b.np * b.nt));
+        const long long o = b.off(i, j, k);
+        double w = 0.0;
+        if (i >= 1 && i <= b.nr - 1) {
+            const double Rf = a.t.Rf[i];
+            const double x = Rf - r, y = (a.t.th[j] - 3.14159265358979323846 / 2.0) * r, z = a.t.ph[k] * r;
+            if (((x * x + y * y) + z * z) < radius) w = (Rf * Rf) * sin(a.t.th[j]);
+        }
+        a.bJ[o] = w;
+        sw += w;
+        swv += w * a.VR[o];
+    }
+    __shared__ double s1[OBJ_BLOCK], s2[OBJ_BLOCK];
+    s1[threadIdx.x] = sw;
+    s2[threadIdx.x] = swv;
+    __syncthreads();
+    for (int h = OBJ_BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s1[threadIdx.x] += s1[threadIdx.x + h];
+            s2[threadIdx.x] += s2[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1[0];
+        part[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+__global__ void __launch_bounds__(OBJ_BLOCK) k_adj_obj_final(const double *part, int nblk, double *out)
+{
+    __shared__ double s1[OBJ_BLOCK], s2[OBJ_BLOCK];
+    double a1 = 0.0, a2 = 0.0;
+    for (int q = threadIdx.x; q < nblk; q += OBJ_BLOCK) {
+        a1 += part[2 * q];
+        a2 += part[2 * q + 1];
+    }
+    s1[threadIdx.x] = a1;
+    s2[threadIdx.x] = a2;
+    __syncthreads();
+    for (int h = OBJ_BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s1[threadIdx.x] += s1[threadIdx.x + h];
+            s2[threadIdx.x] += s2[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s1[0];
+        out[1] = s2[0];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_adj_scale(Box b, double *A, int n1, int n2, int n3, double s)
+{
+    A_LOOP(n1, n2, n3)
+    A[b.off(i, j, k)] *= s;
+}
+
+__global__ void k_adj_zero(double *p, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = 0.0;
+}
+
+} // namespace sspt
+
+// ======================================================================== API
+using namespace sspt;
+
+namespace {
+
+constexpr int OBJ_NBLK = 1024;
+
+int afail(ss_ctx *c, int code, const std::string &msg)
+{
+    c->err = msg;
+    return code;
+}
+
+int async(ss_ctx *c, const char *where)
+{
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        c->err = std::string(where) + ": " + hipGetErrorString(e);
+        return (int)e;
+    }
+    return SS_OK;
+}
+
+int ensure_adj(ss_ctx *c)
+{
+    if (c->adj_ready) return SS_OK;
+    for (int q = 0; q < AD_NBOX; ++q) {
+        if (hipMalloc(&c->adj[q], c->box_elems * sizeof(double)) != hipSuccess)
+            return afail(c, SS_ERR_NOMEM, "adjoint boxes");
+        hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[q], c->box_elems);
+    }
+    if (hipMalloc(&c->adj_part, (2 * OBJ_NBLK + 2) * sizeof(double)) != hipSuccess)
+        return afail(c, SS_ERR_NOMEM, "adjoint partial sums");
+    c->adj_cur = 0;
+    c->adj_ready = true;
+    return SS_OK;
+}
+
+AdjSet set_of(ss_ctx *c, int which)
+{
+    AdjSet s;
+    for (int q = 0; q < L_N; ++q) s.f[q] = c->adj[(which ? AD_SETB : AD_SETA) + q];
+    return s;
+}
+
+AdjArgs make_adj_args(ss_ctx *c, const ss_params &p)
+{
+    static const double PI = 3.14159265358979323846;
+    AdjArgs a{};
+    a.b = c->box;
+    a.t = c->tab;
+    const FieldSet &s = c->cur;
+    a.tRR = s.f[SS_TAU_RR]; a.tTT = s.f[SS_TAU_THTH]; a.tPP = s.f[SS_TAU_PHPH];
+    a.tRT = s.f[SS_TAU_RTH]; a.tRP = s.f[SS_TAU_RPH]; a.tTP = s.f[SS_TAU_THPH];
+    a.eta = s.f[SS_ETA]; a.ini = s.f[SS_ETA_INI]; a.ePL = s.f[SS_ETA_PL]; a.tII = s.f[SS_TII];
+    a.DT = s.f[SS_DT_R]; a.SM = s.f[SS_ETASM]; a.VR = s.f[SS_VR];
+    double **A = c->adj;
+    a.sID = A[AD_SID]; a.sGS = A[AD_SGS]; a.sKB = A[AD_SKB]; a.sAE = A[AD_SAE];
+    a.sX = A[AD_SX]; a.sY = A[AD_SY]; a.sZ = A[AD_SZ]; a.sIT = A[AD_SIT];
+    a.gRT = A[AD_GRT]; a.idRT = A[AD_IDRT]; a.cRT = A[AD_CRT];
+    a.gRP = A[AD_GRP]; a.idRP = A[AD_IDRP]; a.cRP = A[AD_CRP];
+    a.gTP = A[AD_GTP]; a.idTP = A[AD_IDTP]; a.cTP = A[AD_CTP];
+    a.LT = A[AD_LT]; a.Q = A[AD_Q]; a.bJ = nullptr; a.gRho = A[AD_GRHO]; a.gIni = A[AD_GINI];
+    a._dr = p._dr; a._dth = p._dth; a._dph = p._dph; a.g = p.g;
+    a.kb = (((p.dmp * 2.0) * PI) * p.vpdt) / p.lr;
+    a.gs = ((4.0 * PI) * p.vpdt) / p.lr;
+    a.tC = p.tau_C; a.ex = 1.0 - p.n_exp; a.relax = p.relax; a.emin = p.eta_min; a.emax = p.eta_max;
+    a.do_relax = p.relax != 1.0;
+    a.do_lo = p.eta_min > 0.0;
+    a.do_hi = p.eta_max < INFINITY;
+    a.bc_mode = p.bc_mode;
+    a.in = set_of(c, c->adj_cur);
+    a.out = set_of(c, 1 - c->adj_cur);
+    return a;
+}
+
+// lam_out = T^T lam_in (+ bJ on VR if given); gradients when a.grad
+void launch_adj_step(ss_ctx *c, AdjArgs a, bool with_b)
+{
+    const Box &b = c->box;
+    const int m = std::max(std::max(b.nr, b.nt), b.np) + 1;
+    const dim3 B2(64, 4), G2((m + 63) / 64, (m + 3) / 4);
+    hipLaunchKernelGGL(k_adj_bc_ph, G2, B2, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_bc_th, G2, B2, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_bc_R, G2, B2, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    if (a.grad) hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_tii, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_sig_centre, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    for (int fam = 0; fam < 3; ++fam)
+        hipLaunchKernelGGL(k_adj_sig_edge, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a, fam);
+    a.bJ = with_b ? c->adj[AD_BJ] : nullptr;
+    hipLaunchKernelGGL(k_adj_vr, agrid(b.nr + 1, b.nt, b.np), AB, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_vt, agrid(b.nr, b.nt + 1, b.np), AB, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_vp, agrid(b.nr, b.nt, b.np + 1), AB, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_eta, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+}
+
+int adj_field(int field, int *fwd, int *box)
+{
+    static const int map[SS_ADJ_NFIELDS] = {SS_VR, SS_VTH, SS_VPH, SS_P, SS_TAU_RR, SS_TAU_THTH, SS_TAU_PHPH,
+                                            SS_TAU_RTH, SS_TAU_RPH, SS_TAU_THPH, SS_ETA, SS_ETA_PL, SS_RHO,
+                                            SS_ETA_INI};
+    if (field < 0 || field >= SS_ADJ_NFIELDS) return SS_ERR_ARG;
+    *fwd = map[field];
+    *box = field < L_N ? -1 : (field == SS_ADJ_GRAD_RHO ? AD_GRHO : AD_GINI);
+    return SS_OK;
+}
+
+using aclk = std::chrono::steady_clock;
+
+} // namespace
+
+extern "C" {
+
+int ss_adjoint_prepare(ss_ctx *c, const ss_params *p)
+{
+    if (!c || !p) return SS_ERR_ARG;
+    if (!c->grid_set) return afail(c, SS_ERR_STATE, "ss_set_grid was not called");
+    if (hipSetDevice(c->device) != hipSuccess) return afail(c, SS_ERR_STATE, "hipSetDevice");
+    int rc = ensure_adj(c);
+    if (rc != SS_OK) return rc;
+    AdjArgs a = make_adj_args(c, *p);
+    const Box &b = c->box;
+    hipLaunchKernelGGL(k_adj_prep_centre, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    for (int fam = 0; fam < 3; ++fam)
+        hipLaunchKernelGGL(k_adj_prep_edge, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a, fam);
+    return async(c, "ss_adjoint_prepare");
+}
+
+int ss_adjoint_zero(ss_ctx *c)
+{
+    if (!c) return SS_ERR_ARG;
+    if (!c->adj_ready) return afail(c, SS_ERR_STATE, "ss_adjoint_prepare was not called");
+    for (int q = 0; q < AD_PREV; ++q)
+        hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[q], c->box_elems);
+    hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GRHO], c->box_elems);
+    hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GINI], c->box_elems);
+    c->adj_cur = 0;
+    return async(c, "ss_adjoint_zero");
+}
+
+int ss_adjoint_objective(ss_ctx *c, double r, double radius, double *J)
+{
+    if (!c || !J) return SS_ERR_ARG;
+    if (!c->adj_ready) return afail(c, SS_ERR_STATE, "ss_adjoint_prepare was not called");
+    if (hipSetDevice(c->device) != hipSuccess) return afail(c, SS_ERR_STATE, "hipSetDevice");
+    ss_params dummy{};
+    dummy.lr = 1.0;
+    dummy.relax = 1.0;
+    dummy.eta_max = INFINITY;
+    AdjArgs a = make_adj_args(c, dummy);
+    a.bJ = c->adj[AD_BJ];
+    hipLaunchKernelGGL(k_adj_obj_partial, dim3(OBJ_NBLK), dim3(OBJ_BLOCK), 0, c->stream, a, r, radius, c->adj_part);
+    hipLaunchKernelGGL(k_adj_obj_final, dim3(1), dim3(OBJ_BLOCK), 0, c->stream, c->adj_part, OBJ_NBLK,
+                       c->adj_part + 2 * OBJ_NBLK);
+    double sums[2] = {0.0, 0.0};
+    if (hipMemcpyAsync(sums, c->adj_part + 2 * OBJ_NBLK, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+        hipSuccess)
+        return afail(c, SS_ERR_STATE, "objective readback");
+    int rc = async(c, "ss_adjoint_objective");
+    if (rc != SS_OK) return rc;
+    if (!(sums[0] > 0.0)) return afail(c, SS_ERR_ARG, "ss_adjoint_objective: no r-face inside the inclusion");
+    const Box &b = c->box;
+    hipLaunchKernelGGL(k_adj_scale, agrid(b.nr + 1, b.nt, b.np), AB, 0, c->stream, b, c->adj[AD_BJ], b.nr + 1, b.nt,
+                       b.np, 1.0 / sums[0]);
+    *J = sums[1] / sums[0];
+    return async(c, "ss_adjoint_objective");
+}
+
+int ss_adjoint_upload(ss_ctx *c, int field, const double *host, int layout)
+{
+    if (!c || !host || (layout != SS_LAYOUT_JULIA && layout != SS_LAYOUT_C)) return SS_ERR_ARG;
+    if (!c->adj_ready) return afail(c, SS_ERR_STATE, "ss_adjoint_prepare was not called");
+    int fwd, box;
+    if (adj_field(field, &fwd, &box) != SS_OK) return afail(c, SS_ERR_ARG, "ss_adjoint_upload: bad field");
+    int d[3];
+    ss_field_dims(c, fwd, d);
+    const int o1 = (fwd == SS_TAU_RTH || fwd == SS_TAU_RPH || fwd == SS_TAU_THPH) ? 1 : 0;
+    double *dst = box >= 0 ? c->adj[box] : set_of(c, c->adj_cur).f[field];
+    const size_t n = (size_t)d[0] * d[1] * d[2];
+    if (hipMemcpyAsync(c->staging, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return afail(c, SS_ERR_STATE, "ss_adjoint_upload copy");
+    launch_to_box(c, dst, c->staging, d[0], d[1], d[2], o1, layout);
+    return async(c, "ss_adjoint_upload");
+}
+
+int ss_adjoint_download(ss_ctx *c, int field, double *host, int layout)
+{
+    if (!c || !host || (layout != SS_LAYOUT_JULIA && layout != SS_LAYOUT_C)) return SS_ERR_ARG;
+    if (!c->adj_ready) return afail(c, SS_ERR_STATE, "ss_adjoint_prepare was not called");
+    int fwd, box;
+    if (adj_field(field, &fwd, &box) != SS_OK) return afail(c, SS_ERR_ARG, "ss_adjoint_download: bad field");
+    int d[3];
+    ss_field_dims(c, fwd, d);
+    const int o1 = (fwd == SS_TAU_RTH || fwd == SS_TAU_RPH || fwd == SS_TAU_THPH) ? 1 : 0;
+    const double *src = box >= 0 ? c->adj[box] : set_of(c, c->adj_cur).f[field];
+    const size_t n = (size_t)d[0] * d[1] * d[2];
+    launch_from_box(c, c->staging, src, d[0], d[1], d[2], o1, layout);
+    if (hipMemcpyAsync(host, c->staging, n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return afail(c, SS_ERR_STATE, "ss_adjoint_download copy");
+    return async(c, "ss_adjoint_download");
+}
+
+int ss_adjoint_step(ss_ctx *c, const ss_params *p, int gradients)
+{
+    if (!c || !p) return SS_ERR_ARG;
+    if (!c->adj_ready) return afail(c, SS_ERR_STATE, "ss_adjoint_prepare was not called");
+    if (hipSetDevice(c->device) != hipSuccess) return afail(c, SS_ERR_STATE, "hipSetDevice");
+    AdjArgs a = make_adj_args(c, *p);
+    a.grad = gradients ? 1 : 0;
+    launch_adj_step(c, a, false);
+    c->adj_cur = 1 - c->adj_cur;
+    return async(c, "ss_adjoint_step");
+}
+
+int ss_adjoint_solve(ss_ctx *c, const ss_params *p, double r, double radius, int maxiter, int ncheck, double tol,
+                     ss_adjoint_result *out)
+{
+    if (!c || !p || !out || maxiter < 1 || ncheck < 1) return SS_ERR_ARG;
+    memset(out, 0, sizeof *out);
+    out->err = NAN;
+    const aclk::time_point t0 = aclk::now();
+    int rc = ss_adjoint_prepare(c, p);
+    if (rc != SS_OK) return rc;
+    if ((rc = ss_adjoint_zero(c)) != SS_OK) return rc;
+    if ((rc = ss_adjoint_objective(c, r, radius, &out->J)) != SS_OK) return rc;
+    const Box &b = c->box;
+    int it = 0;
+    while (it < maxiter) {
+        const bool check = ((it + 1) % ncheck) == 0;
+        AdjSet cur = set_of(c, c->adj_cur);
+        if (check)
+            for (int q = 0; q < 3; ++q)
+                if (hipMemcpyAsync(c->adj[AD_PREV + q], cur.f[q], c->box_elems * sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                    return afail(c, SS_ERR_STATE, "adjoint check copy");
+        AdjArgs a = make_adj_args(c, *p);
+        launch_adj_step(c, a, true);
+        c->adj_cur = 1 - c->adj_cur;
+        ++it;
+        if (check) {
+            FieldSet s{};
+            AdjSet nw = set_of(c, c->adj_cur);
+            s.f[SS_VR] = nw.f[L_VR];
+            s.f[SS_VTH] = nw.f[L_VT];
+            s.f[SS_VPH] = nw.f[L_VP];
+            launch_check_err(c, s, c->adj[AD_PREV], c->adj[AD_PREV + 1], c->adj[AD_PREV + 2], 0);
+            if (hipMemcpyAsync(c->red_host, c->red, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess)
+                return afail(c, SS_ERR_STATE, "adjoint check readback");
+            if ((rc = async(c, "adjoint iteration")) != SS_OK) return rc;
+            double err = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                double dv, vv;
+                memcpy(&dv, &c->red_host[q], 8);
+                memcpy(&vv, &c->red_host[3 + q], 8);
+                if (std::isnan(dv) || std::isnan(vv)) { out->iters = it; return afail(c, SS_ERR_NONFINITE, "adjoint NaN"); }
+                if (vv > 0.0) err = std::fmax(err, dv / vv);
+            }
+            out->err = err;
+            if (err < tol) { out->converged = 1; break; }
+        }
+    }
+    out->iters = it;
+    // gradients from the converged lam: one more backward pass (its outputs are discarded)
+    AdjArgs a = make_adj_args(c, *p);
+    a.grad = 1;
+    hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GRHO], c->box_elems);
+    hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GINI], c->box_elems);
+    // keep lam: fold and differentiate a copy in the other set
+    AdjSet cur = set_of(c, c->adj_cur), oth = set_of(c, 1 - c->adj_cur);
+    for (int q = 0; q < L_N; ++q)
+        if (hipMemcpyAsync(oth.f[q], cur.f[q], c->box_elems * sizeof(double), hipMemcpyDeviceToDevice, c->stream) !=
+            hipSuccess)
+            return afail(c, SS_ERR_STATE, "adjoint gradient copy");
+    a.in = oth;
+    a.out = oth;
+    const int m = std::max(std::max(b.nr, b.nt), b.np) + 1;
+    const dim3 B2(64, 4), G2((m + 63) / 64, (m + 3) / 4);
+    hipLaunchKernelGGL(k_adj_bc_ph, G2, B2, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_bc_th, G2, B2, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_bc_R, G2, B2, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    if ((rc = async(c, "adjoint gradients")) != SS_OK) return rc;
+    out->seconds = std::chrono::duration<double>(aclk::now() - t0).count();
+    return out->converged ? SS_OK : SS_NOT_CONVERGED;
+}
+
+} // extern "C"

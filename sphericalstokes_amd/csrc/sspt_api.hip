@@ -209,6 +209,9 @@ int ss_destroy(ss_ctx *c)
         if (p) (void)hipFree(p);
     for (float *p : c->f32)
         if (p) (void)hipFree(p);
+    for (double *p : c->adj)
+        if (p) (void)hipFree(p);
+    if (c->adj_part) (void)hipFree(c->adj_part);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);

@@ -72,6 +72,9 @@ struct FieldSet {
 // forcing (divV and the momentum residual of the FP64 iterate).
 enum { SS_F32_ETA = 20, SS_F32_GP = 21, SS_F32_FVR = 22, SS_F32_FVT = 23, SS_F32_FVP = 24, SS_F32_NBOX = 25 };
 
+// FP64 boxes of the adjoint solve (sspt_adjoint.hip lists them)
+enum { SS_ADJ_NBOX = 49 };
+
 } // namespace sspt
 
 struct ss_ctx {
@@ -91,6 +94,10 @@ struct ss_ctx {
     double *tau_scr[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}; // residual scratch
     float *f32[sspt::SS_F32_NBOX] = {};
     bool f32_ready = false;
+    double *adj[sspt::SS_ADJ_NBOX] = {}; // adjoint: two lam sets, check copies, temporaries, fixed coefficients
+    double *adj_part = nullptr;     // partial sums of the objective
+    bool adj_ready = false;
+    int adj_cur = 0;                // lam set holding the current iterate
     bool grid_set = false;
     double *staging = nullptr;      // upload/download staging (largest field)
     size_t staging_elems = 0;

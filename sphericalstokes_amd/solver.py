@@ -239,3 +239,61 @@ class Solver:
         v = C.c_double()
         self._chk(self._L.ss_last_kernel_ms(self._h, C.byref(v)), "last_kernel_ms")
         return v.value
+
+    # ---- adjoint of the converged solve (include/sspt.h, ss_adjoint_*)
+    _ADJ_SHAPE = {"grad_rho": "rho", "grad_eta_ini": "eta_ini"}
+
+    def _adj_id(self, name: str) -> int:
+        if name not in _lib.ADJ_FIELDS:
+            raise ValueError(f"unknown adjoint field {name}")
+        return _lib.ADJ_FIELDS.index(name)
+
+    def _adj_shape(self, name: str):
+        return self.field_shape(self._ADJ_SHAPE.get(name, name))
+
+    def adjoint_prepare(self, p: Params):
+        sp = ss_params.from_params(p)
+        self._chk(self._L.ss_adjoint_prepare(self._h, C.byref(sp)), "adjoint_prepare")
+
+    def adjoint_zero(self):
+        self._chk(self._L.ss_adjoint_zero(self._h), "adjoint_zero")
+
+    def adjoint_objective(self, r: float, radius: float) -> float:
+        J = C.c_double()
+        self._chk(self._L.ss_adjoint_objective(self._h, float(r), float(radius), C.byref(J)), "adjoint_objective")
+        return J.value
+
+    def adjoint_upload(self, name: str, a: np.ndarray):
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape != self._adj_shape(name):
+            raise ValueError(f"{name}: expected shape {self._adj_shape(name)}, got {a.shape}")
+        layout = LAYOUT_JULIA if a.flags.f_contiguous else LAYOUT_C
+        if layout == LAYOUT_C:
+            a = np.ascontiguousarray(a)
+        self._chk(self._L.ss_adjoint_upload(self._h, self._adj_id(name), _dptr(a), layout), f"adjoint_upload {name}")
+
+    def adjoint_download(self, name: str) -> np.ndarray:
+        out = np.empty(self._adj_shape(name), dtype=np.float64, order="F")
+        self._chk(self._L.ss_adjoint_download(self._h, self._adj_id(name), _dptr(out), LAYOUT_JULIA),
+                  f"adjoint_download {name}")
+        return out
+
+    def adjoint_step(self, p: Params, gradients: bool = False):
+        sp = ss_params.from_params(p)
+        self._chk(self._L.ss_adjoint_step(self._h, C.byref(sp), int(bool(gradients))), "adjoint_step")
+
+    def adjoint_solve(self, p: Params, r: float, radius: float, maxiter: int = 4_000_000, ncheck: int = 100,
+                      tol: float = 1e-12):
+        """Gradient of J (volume-weighted mean VR in the inclusion) at the
+        current converged state: dict with J, iterations, and the gradients
+        with respect to rho and log10(eta_ini) (centre arrays)."""
+        sp = ss_params.from_params(p)
+        res = _lib.ss_adjoint_result()
+        rc = self._L.ss_adjoint_solve(self._h, C.byref(sp), float(r), float(radius), int(maxiter), int(ncheck),
+                                      float(tol), C.byref(res))
+        if rc not in (0, _lib.SS_NOT_CONVERGED):
+            self._chk(rc, "adjoint_solve")
+        g_ini = self.adjoint_download("grad_eta_ini")
+        return {"J": res.J, "iters": res.iters, "converged": bool(res.converged), "err": res.err,
+                "seconds": res.seconds, "grad_rho": self.adjoint_download("grad_rho"),
+                "grad_log10_eta_ini": g_ini * self.download("eta_ini") * math.log(10.0)}
