@@ -15,6 +15,9 @@ correction sweep on the same 511^3 workload against one FP64 iteration, and
 the time to reach an FP64 residual of 1e-8 with the plain FP64 solve and
 with the FP32-inner / FP64 defect-correction solve (linear diapir_sector at
 --ttt-n^3, default 63^3 so that the default run stays within minutes).
+"adjoint" holds the cost of one adjoint iteration on the same workload
+against one FP64 iteration, and the wall times of a forward solve and of
+the adjoint solve that gives the gradient (power-law diapir at --adjoint-n^3).
 """
 from __future__ import annotations
 
@@ -68,6 +71,47 @@ def fp32_sweep_ms(s, p, sweeps):
     one outer step of the mixed solve with `sweeps` sweeps and no checks."""
     r = s.pt_solve_mixed(p, max_outer=1, max_inner=sweeps, nout_inner=sweeps + 1, tol_inner=0.0, tol=0.0)
     return r["inner_seconds"] / sweeps * 1e3
+
+
+def adjoint_iteration_ms(s, p, steps, r, radius):
+    """Average wall time of one adjoint iteration (T^T applied to the adjoint
+    fields, ss_adjoint_step) at the state held by s."""
+    s.adjoint_prepare(p)
+    s.adjoint_objective(r, radius)
+    s.adjoint_zero()
+    s.adjoint_step(p)                       # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.adjoint_step(p)
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def adjoint_solve_ratio(n, eps):
+    """Forward solve (reference exit test, relative change of V < eps every
+    500 iterations) and adjoint solve (relative change of the adjoint < eps)
+    of the power-law diapir_sector at n^3, free slip: wall times and ratio."""
+    from sphericalstokes_amd import setups
+    from sphericalstokes_amd.fields import BC_FREE_SLIP, FIELDS
+    from sphericalstokes_amd.solver import Solver
+
+    st = setups.diapir_sector(n, n_exp=3.0, bc_mode=BC_FREE_SLIP, vr=10.0)
+    p = st.params
+    s = Solver(n, n, n)
+    s.set_grid(st.R, st.th, st.ph)
+    for name in FIELDS:
+        s.upload(name, st.fields[name])
+    t0 = time.perf_counter()
+    f = s.pt_solve(p, 4_000_000, 500, eps)
+    tf = time.perf_counter() - t0
+    print(f"  adjoint {n}^3: forward {f['ittot']} iterations in {tf:.2f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    a = s.adjoint_solve(p, 1.5, 0.1 ** 2, maxiter=4_000_000, ncheck=500, tol=eps)
+    ta = time.perf_counter() - t0
+    s.close()
+    return {"workload": f"diapir_sector {n}^3, n_exp=3, free slip; J = mean VR in the inclusion", "eps": eps,
+            "forward_s": round(tf, 3), "forward_iters": f["ittot"], "adjoint_s": round(ta, 3),
+            "adjoint_iters": a["iters"], "adjoint_converged": a["converged"],
+            "ratio_adjoint_over_forward": round(ta / tf, 4)}
 
 
 def time_to_tolerance(n, tol, tol_inner):
@@ -141,6 +185,10 @@ def main():
     ap.add_argument("--ttt-tol", type=float, default=1e-8)
     ap.add_argument("--fp32-sweeps", type=int, default=20,
                     help="FP32 correction sweeps timed on the main workload; 0 skips them")
+    ap.add_argument("--adjoint-steps", type=int, default=10,
+                    help="adjoint iterations timed on the main workload; 0 skips them")
+    ap.add_argument("--adjoint-n", type=int, default=63,
+                    help="size of the forward-vs-adjoint solve comparison; 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain field copy (k_copy_field: n^3 doubles read + written) so that "
                          "rocprofv3 --pmc byte counters can be scaled")
@@ -167,6 +215,8 @@ def main():
     main_ms, iter_ms = s.pt_iterate_timed(p, args.steps)
     wall = time.perf_counter() - t0
     sweep_ms = fp32_sweep_ms(s, p, args.fp32_sweeps) if args.fp32_sweeps > 0 else None
+    # adjoint iteration on the same state (J: mean VR in a sphere of radius 0.01 in the crust)
+    adj_ms = adjoint_iteration_ms(s, p, args.adjoint_steps, 0.985, 0.01 ** 2) if args.adjoint_steps > 0 else None
     s.close()
     ms_per_step = wall / args.steps * 1e3
     A = a_eff_bytes(n)
@@ -200,6 +250,9 @@ def main():
             "fp32_sweep_over_fp64_iter": round(sweep_ms / ms_per_step, 4) if sweep_ms else None,
             "time_to_tol": None},
     }
+    out["adjoint"] = {"iter_ms": round(adj_ms, 4) if adj_ms else None,
+                      "iter_over_fp64_iter": round(adj_ms / ms_per_step, 4) if adj_ms else None,
+                      "solve": adjoint_solve_ratio(args.adjoint_n, 1e-10) if args.adjoint_n > 0 else None}
     if args.ttt_n > 0:
         out["mixed_precision"]["time_to_tol"] = time_to_tolerance(args.ttt_n, args.ttt_tol, 1e-5)
     if not args.no_cpu:

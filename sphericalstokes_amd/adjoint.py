@@ -1,0 +1,82 @@
+"""Differentiable ascent velocity: the forward PT solve and its adjoint as a
+torch.autograd.Function (the counterpart of the ChainRulesCore rrule a Julia
+caller would write over the same C ABI).
+
+J(rho, log10 eta_ini) = volume-weighted mean VR over the r-faces inside the
+inclusion of the converged PT solve.  forward runs the fused PT loop with the
+reference exit test (ss_pt_solve) and keeps the converged state on the GPU;
+backward runs the adjoint solve on that state (ss_adjoint_solve): no record
+of the forward iterations is kept.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .fields import FIELDS
+from .setups import Setup
+from .solver import Solver
+
+
+@dataclass
+class SolveOptions:
+    maxiter: int = 4_000_000
+    nout: int = 500
+    eps: float = 1e-12          # reference exit test: relative change of V per check
+    r: float = 1.5               # inclusion centre radius (diapir_sector)
+    radius: float = 0.1 ** 2     # squared inclusion radius, as initialize_inclusion! takes it
+    adj_maxiter: int = 4_000_000
+    adj_ncheck: int = 200
+    adj_tol: float = 1e-13
+    info: dict = field(default_factory=dict)
+
+
+def _solver_with(setup: Setup, rho: np.ndarray, log10_ini: np.ndarray) -> Solver:
+    s = Solver(setup.nr, setup.nth, setup.nph)
+    s.set_grid(setup.R, setup.th, setup.ph)
+    ini = np.asfortranarray(10.0 ** log10_ini)
+    f = dict(setup.fields)
+    f.update({"rho": np.asfortranarray(rho), "eta_ini": ini, "eta": ini, "eta_PL": ini})
+    for n in FIELDS:
+        s.upload(n, f[n])
+    return s
+
+
+class AscentVelocity(torch.autograd.Function):
+    """J = AscentVelocity.apply(rho, log10_eta_ini, setup, options); rho and
+    log10_eta_ini are float64 tensors of the centre shape (nr, nth, nph)."""
+
+    @staticmethod
+    def forward(ctx, rho, log10_eta_ini, setup: Setup, opts: SolveOptions):
+        s = _solver_with(setup, rho.detach().cpu().numpy().astype(np.float64),
+                         log10_eta_ini.detach().cpu().numpy().astype(np.float64))
+        res = s.pt_solve(setup.params, opts.maxiter, opts.nout, opts.eps)
+        s.adjoint_prepare(setup.params)
+        J = s.adjoint_objective(opts.r, opts.radius)
+        opts.info["forward"] = res
+        if not any(ctx.needs_input_grad[:2]):      # no backward will come: release the device state now
+            s.close()
+            s = None
+        ctx.solver, ctx.setup, ctx.opts = s, setup, opts
+        ctx.device, ctx.dtype = rho.device, rho.dtype
+        return torch.tensor(J, dtype=torch.float64, device=rho.device)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        s, setup, o = ctx.solver, ctx.setup, ctx.opts
+        try:
+            r = s.adjoint_solve(setup.params, o.r, o.radius, o.adj_maxiter, o.adj_ncheck, o.adj_tol)
+        finally:
+            s.close()
+            ctx.solver = None
+        o.info["adjoint"] = {k: v for k, v in r.items() if not k.startswith("grad")}
+        g = float(gJ)
+        to = lambda a: torch.as_tensor(np.ascontiguousarray(a) * g, dtype=ctx.dtype, device=ctx.device)
+        return to(r["grad_rho"]), to(r["grad_log10_eta_ini"]), None, None
+
+
+def ascent_velocity(rho, log10_eta_ini, setup: Setup, opts: SolveOptions | None = None):
+    return AscentVelocity.apply(rho, log10_eta_ini, setup, opts or SolveOptions())
+

@@ -129,3 +129,35 @@ def test_adjoint_gradient_central_differences(Solver, n_exp):
         ad = float(np.sum((r["grad_rho"] if what == "rho" else r["grad_log10_eta_ini"]) * dm))
         say(what, "fd", fd, "adjoint", ad)
         assert abs(fd - ad) <= 1e-5 * abs(fd), (what, fd, ad)
+
+
+def test_autograd_function_matches_central_differences():
+    """J.backward() of the torch.autograd.Function (forward PT solve, adjoint
+    backward) against central differences of the same forward."""
+    import torch
+    from sphericalstokes_amd.adjoint import SolveOptions, ascent_velocity
+
+    st = setups.diapir_sector(13, n_exp=3.0, bc_mode=BC_FREE_SLIP, vr=10.0)
+    rho0 = np.ascontiguousarray(st.fields["rho"])
+    le0 = np.log10(np.ascontiguousarray(st.fields["eta_ini"]))
+    opts = SolveOptions(eps=1e-13, nout=500)
+    rho = torch.tensor(rho0, requires_grad=True)
+    le = torch.tensor(le0, requires_grad=True)
+    J = ascent_velocity(rho, le, st, opts)
+    J.backward()
+    say("autograd", float(J), opts.info)
+    rng = np.random.default_rng(7)
+    h = 1e-6
+    for name, grad, x0 in (("rho", rho.grad.numpy(), rho0), ("log10_eta_ini", le.grad.numpy(), le0)):
+        dm = rng.standard_normal(x0.shape)
+        Js = []
+        for sgn in (+1, -1):
+            args = [torch.tensor(rho0), torch.tensor(le0)]
+            k = 0 if name == "rho" else 1
+            args[k] = torch.tensor(x0 + sgn * h * dm)
+            with torch.no_grad():
+                Js.append(float(ascent_velocity(args[0], args[1], st, SolveOptions(eps=1e-13, nout=500))))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float(np.sum(grad * dm))
+        say(name, "fd", fd, "autograd", ad)
+        assert abs(fd - ad) <= 1e-5 * abs(fd), (name, fd, ad)
