@@ -20,10 +20,11 @@
 //   2. power law: lam_tII of every cell from lam_eta (and lam_eta_PL);
 //   3. interior cells: Q = (sum of lam_tII over the cells that copy_BC!
 //      filled from them) / tII;
-//   4. centres and edges: adjoints of the new P and tau (compute_V!'s and
-//      compute_TII!'s transposes), written scaled by compute_P!'s 1/den;
-//   5. faces and centres: lam_V (compute_P!'s strain-rate transposes) and
-//      lam_eta (its viscosity dependence).
+//   4. centres and edges (one kernel): adjoints of the new P and tau
+//      (compute_V!'s and compute_TII!'s transposes), written scaled by
+//      compute_P!'s 1/den;
+//   5. faces and centres (one kernel): lam_V (compute_P!'s strain-rate
+//      transposes) and lam_eta (its viscosity dependence).
 #include "sspt_internal.h"
 #include "sspt_launch.h"
 #include <chrono>
@@ -312,107 +313,111 @@ __global__ void __launch_bounds__(256) k_adj_tii(AdjArgs a)
 }
 
 // ------------------------------------------------------------------ 4. new-stress adjoints
-// centres: P, tau_nn
-__global__ void __launch_bounds__(256) k_adj_sig_centre(AdjArgs a)
+// One thread per box index (i,j,k): the centre (i,j,k) and the three edges
+// stored at (i,j,k) share the face weights they gather.  The adjoints of the
+// new P and tau (compute_V!'s and compute_TII!'s transposes) are written
+// scaled by compute_P!'s 1/den: the out set holds lam_P' and lam_tau'/den.
+__global__ void __launch_bounds__(256) k_adj_sig(AdjArgs a)
 {
     A_LOOP(a.b.nr, a.b.nt, a.b.np)
     const Box &b = a.b;
     const Tables &t = a.t;
     const int nt = b.nt;
     const long long o = b.off(i, j, k);
-    const double wR0 = w_VR(a, i, j, k), wR1 = w_VR(a, i + 1, j, k);
-    const double wT0 = w_VT(a, i, j, k), wT1 = w_VT(a, i, j + 1, k);
-    const double wP0 = w_VP(a, i, j, k), wP1 = w_VP(a, i, j, k + 1);
-    double sRR = 0.0, sTT = 0.0, sPP = 0.0;
-    if (wR0 != 0.0) {
+    const bool eRT = in_RT(b, i, j, k), eRP = in_RP(b, i, j, k), eTP = in_TP(b, i, j, k);
+    // face weights (0 outside compute_V!'s ranges)
+    const double wR = w_VR(a, i, j, k), wRi = w_VR(a, i + 1, j, k), wRj = w_VR(a, i, j - 1, k),
+                 wRk = w_VR(a, i, j, k - 1);
+    const double wT = w_VT(a, i, j, k), wTj = w_VT(a, i, j + 1, k), wTi = w_VT(a, i - 1, j, k),
+                 wTk = w_VT(a, i, j, k - 1);
+    const double wP = w_VP(a, i, j, k), wPk = w_VP(a, i, j, k + 1), wPi = w_VP(a, i - 1, j, k),
+                 wPj = w_VP(a, i, j - 1, k);
+    double vP = 0.0, vRR = 0.0, vTT = 0.0, vPP = 0.0, vRT = 0.0, vRP = 0.0, vTP = 0.0;
+    { // the centre (every (i,j,k) of this grid is one)
+        double sRR = 0.0, sTT = 0.0, sPP = 0.0;
+        if (wR != 0.0) {
+            const double iR = 1.0 / t.Rf[i];
+            sRR += wR * (a._dr + iR);
+            sTT -= 0.5 * wR * iR;
+            sPP -= 0.5 * wR * iR;
+        }
+        if (wRi != 0.0) {
+            const double iR = 1.0 / t.Rf[i + 1];
+            sRR += wRi * (-a._dr + iR);
+            sTT -= 0.5 * wRi * iR;
+            sPP -= 0.5 * wRi * iR;
+        }
+        const double iRy = 1.0 / t.Rc2[i];
+        if (wT != 0.0) {
+            const double ct = t.cotf2[j];
+            sTT += wT * (a._dth * iRy + 0.5 * ct * iRy);
+            sPP -= 0.5 * wT * ct * iRy;
+        }
+        if (wTj != 0.0) {
+            const double ct = t.cotf2[j + 1];
+            sTT += wTj * (-a._dth * iRy + 0.5 * ct * iRy);
+            sPP -= 0.5 * wTj * ct * iRy;
+        }
+        sPP += (wP - wPk) * (t.rs_z[i * (nt + 1) + j] * a._dph);
+        double nRR = a.in.f[L_RR][o] + sRR, nTT = a.in.f[L_TT][o] + sTT, nPP = a.in.f[L_PP][o] + sPP;
+        if (inner_c(b, i, j, k)) {
+            const double q = 0.5 * a.Q[o];
+            nRR += q * a.tRR[o];
+            nTT += q * a.tTT[o];
+            nPP += q * a.tPP[o];
+        }
+        vP = a.in.f[L_P][o] - ((sRR + sTT) + sPP);
+        const double id = a.sID[o];
+        vRR = nRR * id;
+        vTT = nTT * id;
+        vPP = nPP * id;
+    }
+    if (eRT) {
         const double iR = 1.0 / t.Rf[i];
-        sRR += wR0 * (a._dr + iR);
-        sTT -= 0.5 * wR0 * iR;
-        sPP -= 0.5 * wR0 * iR;
-    }
-    if (wR1 != 0.0) {
-        const double iR = 1.0 / t.Rf[i + 1];
-        sRR += wR1 * (-a._dr + iR);
-        sTT -= 0.5 * wR1 * iR;
-        sPP -= 0.5 * wR1 * iR;
-    }
-    const double iRy = 1.0 / t.Rc2[i];
-    if (wT0 != 0.0) {
-        const double ct = t.cotf2[j];
-        sTT += wT0 * (a._dth * iRy + 0.5 * ct * iRy);
-        sPP -= 0.5 * wT0 * ct * iRy;
-    }
-    if (wT1 != 0.0) {
-        const double ct = t.cotf2[j + 1];
-        sTT += wT1 * (-a._dth * iRy + 0.5 * ct * iRy);
-        sPP -= 0.5 * wT1 * ct * iRy;
-    }
-    const double rz = t.rs_z[i * (nt + 1) + j] * a._dph;
-    sPP += (wP0 - wP1) * rz;
-    double nRR = a.in.f[L_RR][o] + sRR, nTT = a.in.f[L_TT][o] + sTT, nPP = a.in.f[L_PP][o] + sPP;
-    if (inner_c(b, i, j, k)) {
-        const double q = 0.5 * a.Q[o];
-        nRR += q * a.tRR[o];
-        nTT += q * a.tTT[o];
-        nPP += q * a.tPP[o];
-    }
-    a.out.f[L_P][o] = a.in.f[L_P][o] - ((sRR + sTT) + sPP);
-    const double id = a.sID[o];
-    a.out.f[L_RR][o] = nRR * id;
-    a.out.f[L_TT][o] = nTT * id;
-    a.out.f[L_PP][o] = nPP * id;
-}
-
-__global__ void __launch_bounds__(256) k_adj_sig_edge(AdjArgs a, int fam)
-{
-    A_LOOP(a.b.nr, a.b.nt, a.b.np)
-    const Box &b = a.b;
-    const Tables &t = a.t;
-    const int nt = b.nt;
-    const long long o = b.off(i, j, k);
-    double s = 0.0, tq = 0.0;
-    if (fam == 0) {
-        if (!in_RT(b, i, j, k)) return;
-        const double iR = 1.0 / t.Rf[i];
-        s += w_VR(a, i, j - 1, k) * (a._dth * iR + 0.5 * t.cotc2[j - 1] * iR);
-        s += w_VR(a, i, j, k) * (-a._dth * iR + 0.5 * t.cotc2[j] * iR);
-        s += w_VT(a, i - 1, j, k) * (a._dr + 1.5 / t.Rc2[i - 1]);
-        s += w_VT(a, i, j, k) * (-a._dr + 1.5 / t.Rc2[i]);
+        double s = wRj * (a._dth * iR + 0.5 * t.cotc2[j - 1] * iR) + wR * (-a._dth * iR + 0.5 * t.cotc2[j] * iR);
+        s += wTi * (a._dr + 1.5 / t.Rc2[i - 1]) + wT * (-a._dr + 1.5 / t.Rc2[i]);
+        double tq = 0.0;
         for (int di = 0; di <= 1; ++di)
             for (int dj = 0; dj <= 1; ++dj)
                 if (inner_c(b, i - di, j - dj, k)) {
                     const long long c = o - di * b.si - dj * b.sj;
                     tq += a.Q[c] * a.sX[c];
                 }
-        a.out.f[L_RT][o] = ((a.in.f[L_RT][o] + s) + tq) * a.idRT[o];
-    } else if (fam == 1) {
-        if (!in_RP(b, i, j, k)) return;
-        const double rf = t.rs_f[i * (nt + 1) + j] * a._dph;
-        s += (w_VR(a, i, j, k - 1) - w_VR(a, i, j, k)) * rf;
-        s += w_VP(a, i - 1, j, k) * (a._dr + 1.5 / t.Rc2[i - 1]);
-        s += w_VP(a, i, j, k) * (-a._dr + 1.5 / t.Rc2[i]);
+        vRT = ((a.in.f[L_RT][o] + s) + tq) * a.idRT[o];
+    }
+    if (eRP) {
+        double s = (wRk - wR) * (t.rs_f[i * (nt + 1) + j] * a._dph);
+        s += wPi * (a._dr + 1.5 / t.Rc2[i - 1]) + wP * (-a._dr + 1.5 / t.Rc2[i]);
+        double tq = 0.0;
         for (int di = 0; di <= 1; ++di)
             for (int dk = 0; dk <= 1; ++dk)
                 if (inner_c(b, i - di, j, k - dk)) {
                     const long long c = o - di * b.si - dk;
                     tq += a.Q[c] * a.sY[c];
                 }
-        a.out.f[L_RP][o] = ((a.in.f[L_RP][o] + s) + tq) * a.idRP[o];
-    } else {
-        if (!in_TP(b, i, j, k)) return;
-        const double ry = t.rs_y[i * (nt + 1) + j] * a._dph;
-        s += (w_VT(a, i, j, k - 1) - w_VT(a, i, j, k)) * ry;
+        vRP = ((a.in.f[L_RP][o] + s) + tq) * a.idRP[o];
+    }
+    if (eTP) {
+        double s = (wTk - wT) * (t.rs_y[i * (nt + 1) + j] * a._dph);
         const double iR = 1.0 / t.Rc2[i];
-        s += w_VP(a, i, j - 1, k) * (a._dth * iR + t.cotc2[j - 1] * iR);
-        s += w_VP(a, i, j, k) * (-a._dth * iR + t.cotc2[j] * iR);
+        s += wPj * (a._dth * iR + t.cotc2[j - 1] * iR) + wP * (-a._dth * iR + t.cotc2[j] * iR);
+        double tq = 0.0;
         for (int dj = 0; dj <= 1; ++dj)
             for (int dk = 0; dk <= 1; ++dk)
                 if (inner_c(b, i, j - dj, k - dk)) {
                     const long long c = o - dj * b.sj - dk;
                     tq += a.Q[c] * a.sZ[c];
                 }
-        a.out.f[L_TP][o] = ((a.in.f[L_TP][o] + s) + tq) * a.idTP[o];
+        vTP = ((a.in.f[L_TP][o] + s) + tq) * a.idTP[o];
     }
+    // stores last (the loads above may then be scheduled freely)
+    a.out.f[L_P][o] = vP;
+    a.out.f[L_RR][o] = vRR;
+    a.out.f[L_TT][o] = vTT;
+    a.out.f[L_PP][o] = vPP;
+    if (eRT) a.out.f[L_RT][o] = vRT;
+    if (eRP) a.out.f[L_RP][o] = vRP;
+    if (eTP) a.out.f[L_TP][o] = vTP;
 }
 
 // ------------------------------------------------------------------ 5. lam_V and lam_eta
@@ -437,112 +442,103 @@ __device__ __forceinline__ CW centre_w(const AdjArgs &a, long long c, int i, int
     return w;
 }
 
-__global__ void __launch_bounds__(256) k_adj_vr(AdjArgs a)
+// One thread per box index: VR, Vth and Vph faces (i,j,k) and the centre
+// viscosity (i,j,k).  Each face takes compute_P!'s centre stencil from the
+// cell above and below it and the shear-edge stencils around it.
+__global__ void __launch_bounds__(256) k_adj_vel(AdjArgs a)
 {
-    A_LOOP(a.b.nr + 1, a.b.nt, a.b.np)
+    A_LOOP(a.b.nr + 1, a.b.nt + 1, a.b.np + 1)
     const Box &b = a.b;
     const Tables &t = a.t;
-    const int nt = b.nt;
+    const int nr = b.nr, nt = b.nt, np = b.np;
     const long long o = b.off(i, j, k);
-    double v = a.in.f[L_VR][o];
-    if (i >= 1) { // upper face of cell i-1
-        const CW w = centre_w(a, o - b.si, i - 1, j);
-        v += w.dR * a._dr + 0.5 * w.aR;
-    }
-    if (i <= b.nr - 1) { // lower face of cell i
-        const CW w = centre_w(a, o, i, j);
-        v += -w.dR * a._dr + 0.5 * w.aR;
-    }
-    if (in_RT(b, i, j, k)) v += a.gRT[o] * a.out.f[L_RT][o] * a._dth / t.Re[i];
-    if (in_RT(b, i, j + 1, k)) v -= a.gRT[o + b.sj] * a.out.f[L_RT][o + b.sj] * a._dth / t.Re[i];
-    if (in_RP(b, i, j, k) || in_RP(b, i, j, k + 1)) {
-        const double re = t.rs_e[i * (nt + 1) + j] * a._dph;
-        if (in_RP(b, i, j, k)) v += a.gRP[o] * a.out.f[L_RP][o] * re;
-        if (in_RP(b, i, j, k + 1)) v -= a.gRP[o + 1] * a.out.f[L_RP][o + 1] * re;
-    }
-    if (a.bJ) v += a.bJ[o];
-    a.out.f[L_VR][o] = v;
-}
-
-__global__ void __launch_bounds__(256) k_adj_vt(AdjArgs a)
-{
-    A_LOOP(a.b.nr, a.b.nt + 1, a.b.np)
-    const Box &b = a.b;
-    const Tables &t = a.t;
-    const int nt = b.nt;
-    const long long o = b.off(i, j, k);
-    double v = a.in.f[L_VT][o];
-    const double dth_R = a._dth * t.invRc[i];
-    if (j >= 1) { // upper th-face of cell j-1
-        const CW w = centre_w(a, o - b.sj, i, j - 1);
-        v += w.dT * dth_R + 0.5 * w.aT;
-    }
-    if (j <= nt - 1) {
-        const CW w = centre_w(a, o, i, j);
-        v += -w.dT * dth_R + 0.5 * w.aT;
-    }
-    if (in_RT(b, i, j, k)) v += a.gRT[o] * a.out.f[L_RT][o] * (a._dr - 0.5 / t.Re[i]);
-    if (in_RT(b, i + 1, j, k)) v += a.gRT[o + b.si] * a.out.f[L_RT][o + b.si] * (-a._dr - 0.5 / t.Re[i + 1]);
-    if (in_TP(b, i, j, k) || in_TP(b, i, j, k + 1)) {
-        const double r4 = t.rs_c4[i * (nt + 1) + j] * a._dph;
-        if (in_TP(b, i, j, k)) v += a.gTP[o] * a.out.f[L_TP][o] * r4;
-        if (in_TP(b, i, j, k + 1)) v -= a.gTP[o + 1] * a.out.f[L_TP][o + 1] * r4;
-    }
-    a.out.f[L_VT][o] = v;
-}
-
-__global__ void __launch_bounds__(256) k_adj_vp(AdjArgs a)
-{
-    A_LOOP(a.b.nr, a.b.nt, a.b.np + 1)
-    const Box &b = a.b;
-    const Tables &t = a.t;
-    const int nt = b.nt;
-    const long long o = b.off(i, j, k);
-    double v = a.in.f[L_VP][o];
-    const double rc = t.rs_c[i * (nt + 1) + j] * a._dph;
-    if (k >= 1) {
-        const CW w = centre_w(a, o - 1, i, j);
-        v += w.dP * rc;
-    }
-    if (k <= b.np - 1) {
-        const CW w = centre_w(a, o, i, j);
-        v -= w.dP * rc;
-    }
-    if (in_RP(b, i, j, k)) v += a.gRP[o] * a.out.f[L_RP][o] * (a._dr - 0.5 / t.Re[i]);
-    if (in_RP(b, i + 1, j, k)) v += a.gRP[o + b.si] * a.out.f[L_RP][o + b.si] * (-a._dr - 0.5 / t.Re[i + 1]);
-    if (in_TP(b, i, j, k)) {
-        const double iR = 1.0 / t.Rc4[i];
-        v += a.gTP[o] * a.out.f[L_TP][o] * (a._dth * iR - 0.5 * t.cotf4[j] * iR);
-    }
-    if (in_TP(b, i, j + 1, k)) {
-        const double iR = 1.0 / t.Rc4[i];
-        v += a.gTP[o + b.sj] * a.out.f[L_TP][o + b.sj] * (-a._dth * iR - 0.5 * t.cotf4[j + 1] * iR);
-    }
-    a.out.f[L_VP][o] = v;
-}
-
-__global__ void __launch_bounds__(256) k_adj_eta(AdjArgs a)
-{
-    A_LOOP(a.b.nr, a.b.nt, a.b.np)
-    const Box &b = a.b;
-    const long long o = b.off(i, j, k);
-    double v = ((a.out.f[L_RR][o] * a.tRR[o] + a.out.f[L_TT][o] * a.tTT[o]) + a.out.f[L_PP][o] * a.tPP[o]) * a.sAE[o];
-    for (int d1 = 0; d1 <= 1; ++d1)
-        for (int d2 = 0; d2 <= 1; ++d2) {
-            if (in_RT(b, i + d1, j + d2, k)) {
-                const long long e = o + d1 * b.si + d2 * b.sj;
-                v += a.cRT[e] * a.out.f[L_RT][e];
-            }
-            if (in_RP(b, i + d1, j, k + d2)) {
-                const long long e = o + d1 * b.si + d2;
-                v += a.cRP[e] * a.out.f[L_RP][e];
-            }
-            if (in_TP(b, i, j + d1, k + d2)) {
-                const long long e = o + d1 * b.sj + d2;
-                v += a.cTP[e] * a.out.f[L_TP][e];
-            }
+    const bool fR = j < nt && k < np, fT = i < nr && k < np, fP = i < nr && j < nt, cen = i < nr && j < nt && k < np;
+    CW w0{0, 0, 0, 0, 0};
+    if (cen) w0 = centre_w(a, o, i, j);
+    // shear-edge terms g * lam_out of the edges around
+    const bool rt = in_RT(b, i, j, k), rp = in_RP(b, i, j, k), tp = in_TP(b, i, j, k);
+    const double sRT = rt ? a.gRT[o] * a.out.f[L_RT][o] : 0.0;
+    const double sRP = rp ? a.gRP[o] * a.out.f[L_RP][o] : 0.0;
+    const double sTP = tp ? a.gTP[o] * a.out.f[L_TP][o] : 0.0;
+    double vR = 0.0, vT = 0.0, vP = 0.0, vE = 0.0;
+    if (fR) {
+        double v = a.in.f[L_VR][o];
+        if (i >= 1) {
+            const CW w = centre_w(a, o - b.si, i - 1, j);
+            v += w.dR * a._dr + 0.5 * w.aR;
         }
-    a.out.f[L_E][o] = v;
+        if (i <= nr - 1) v += -w0.dR * a._dr + 0.5 * w0.aR;
+        if (rt) v += sRT * a._dth / t.Re[i];
+        if (in_RT(b, i, j + 1, k)) v -= a.gRT[o + b.sj] * a.out.f[L_RT][o + b.sj] * a._dth / t.Re[i];
+        if (rp || in_RP(b, i, j, k + 1)) {
+            const double re = t.rs_e[i * (nt + 1) + j] * a._dph;
+            v += sRP * re;
+            if (in_RP(b, i, j, k + 1)) v -= a.gRP[o + 1] * a.out.f[L_RP][o + 1] * re;
+        }
+        if (a.bJ) v += a.bJ[o];
+        vR = v;
+    }
+    if (fT) {
+        double v = a.in.f[L_VT][o];
+        const double dth_R = a._dth * t.invRc[i];
+        if (j >= 1) {
+            const CW w = centre_w(a, o - b.sj, i, j - 1);
+            v += w.dT * dth_R + 0.5 * w.aT;
+        }
+        if (j <= nt - 1) v += -w0.dT * dth_R + 0.5 * w0.aT;
+        if (rt) v += sRT * (a._dr - 0.5 / t.Re[i]);
+        if (in_RT(b, i + 1, j, k)) v += a.gRT[o + b.si] * a.out.f[L_RT][o + b.si] * (-a._dr - 0.5 / t.Re[i + 1]);
+        if (tp || in_TP(b, i, j, k + 1)) {
+            const double r4 = t.rs_c4[i * (nt + 1) + j] * a._dph;
+            v += sTP * r4;
+            if (in_TP(b, i, j, k + 1)) v -= a.gTP[o + 1] * a.out.f[L_TP][o + 1] * r4;
+        }
+        vT = v;
+    }
+    if (fP) {
+        double v = a.in.f[L_VP][o];
+        const double rc = t.rs_c[i * (nt + 1) + j] * a._dph;
+        if (k >= 1) {
+            const CW w = centre_w(a, o - 1, i, j);
+            v += w.dP * rc;
+        }
+        if (k <= np - 1) v -= w0.dP * rc;
+        if (rp) v += sRP * (a._dr - 0.5 / t.Re[i]);
+        if (in_RP(b, i + 1, j, k)) v += a.gRP[o + b.si] * a.out.f[L_RP][o + b.si] * (-a._dr - 0.5 / t.Re[i + 1]);
+        if (tp) {
+            const double iR = 1.0 / t.Rc4[i];
+            v += sTP * (a._dth * iR - 0.5 * t.cotf4[j] * iR);
+        }
+        if (in_TP(b, i, j + 1, k)) {
+            const double iR = 1.0 / t.Rc4[i];
+            v += a.gTP[o + b.sj] * a.out.f[L_TP][o + b.sj] * (-a._dth * iR - 0.5 * t.cotf4[j + 1] * iR);
+        }
+        vP = v;
+    }
+    if (cen) {
+        double v = ((a.out.f[L_RR][o] * a.tRR[o] + a.out.f[L_TT][o] * a.tTT[o]) + a.out.f[L_PP][o] * a.tPP[o]) *
+                   a.sAE[o];
+        for (int d1 = 0; d1 <= 1; ++d1)
+            for (int d2 = 0; d2 <= 1; ++d2) {
+                if (in_RT(b, i + d1, j + d2, k)) {
+                    const long long e = o + d1 * b.si + d2 * b.sj;
+                    v += a.cRT[e] * a.out.f[L_RT][e];
+                }
+                if (in_RP(b, i + d1, j, k + d2)) {
+                    const long long e = o + d1 * b.si + d2;
+                    v += a.cRP[e] * a.out.f[L_RP][e];
+                }
+                if (in_TP(b, i, j + d1, k + d2)) {
+                    const long long e = o + d1 * b.sj + d2;
+                    v += a.cTP[e] * a.out.f[L_TP][e];
+                }
+            }
+        vE = v;
+    }
+    if (fR) a.out.f[L_VR][o] = vR;
+    if (fT) a.out.f[L_VT][o] = vT;
+    if (fP) a.out.f[L_VP][o] = vP;
+    if (cen) a.out.f[L_E][o] = vE;
 }
 
 // gradient with respect to rho: compute_V!'s -rho g on VR faces
@@ -718,14 +714,9 @@ void launch_adj_step(ss_ctx *c, AdjArgs a, bool with_b)
     hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
     if (a.grad) hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
     hipLaunchKernelGGL(k_adj_tii, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_sig_centre, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
-    for (int fam = 0; fam < 3; ++fam)
-        hipLaunchKernelGGL(k_adj_sig_edge, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a, fam);
+    hipLaunchKernelGGL(k_adj_sig, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
     a.bJ = with_b ? c->adj[AD_BJ] : nullptr;
-    hipLaunchKernelGGL(k_adj_vr, agrid(b.nr + 1, b.nt, b.np), AB, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_vt, agrid(b.nr, b.nt + 1, b.np), AB, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_vp, agrid(b.nr, b.nt, b.np + 1), AB, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_eta, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_vel, agrid(b.nr + 1, b.nt + 1, b.np + 1), AB, 0, c->stream, a);
 }
 
 int adj_field(int field, int *fwd, int *box)
