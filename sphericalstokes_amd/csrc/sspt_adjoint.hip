@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) k_adj_power_law(AdjArgs a)
     if (!isfinite(pl) || !isfinite(pl0)) { // tII = 0: eta = 2 eta_ini
         a.LT[o] = 0.0;
         if (a.grad) a.gIni[o] += a_e * 2.0;
-        a.out.f[L_PL][o] = 0.0;
+        if (a.do_relax) a.out.f[L_PL][o] = 0.0;
         return;
     }
     double l_pl = a_e * (e_raw * e_raw) / (2.0 * pl * pl);
@@ -279,9 +279,7 @@ __global__ void __launch_bounds__(256) k_adj_power_law(AdjArgs a)
         const double l_pl0 = l_pl * a.relax * pl / pl0;
         a.out.f[L_PL][o] = l_pl * (1.0 - a.relax) * pl / a.ePL[o];
         l_pl = l_pl0;
-    } else {
-        a.out.f[L_PL][o] = 0.0;
-    }
+    } // without relaxation eta_PL is no state: its adjoint stays 0 in both sets
     a.LT[o] = (tii > 0.0) ? l_pl * a.ex * pl0 / tii : 0.0;
     l_ini += l_pl * pl0 / ini;
     if (a.grad) a.gIni[o] += l_ini;
@@ -317,9 +315,8 @@ __global__ void __launch_bounds__(256) k_adj_tii(AdjArgs a)
 // stored at (i,j,k) share the face weights they gather.  The adjoints of the
 // new P and tau (compute_V!'s and compute_TII!'s transposes) are written
 // scaled by compute_P!'s 1/den: the out set holds lam_P' and lam_tau'/den.
-__global__ void __launch_bounds__(256) k_adj_sig(AdjArgs a)
+__device__ __forceinline__ void adj_sig_point(const AdjArgs &a, const int i, const int j, const int k)
 {
-    A_LOOP(a.b.nr, a.b.nt, a.b.np)
     const Box &b = a.b;
     const Tables &t = a.t;
     const int nt = b.nt;
@@ -445,9 +442,8 @@ __device__ __forceinline__ CW centre_w(const AdjArgs &a, long long c, int i, int
 // One thread per box index: VR, Vth and Vph faces (i,j,k) and the centre
 // viscosity (i,j,k).  Each face takes compute_P!'s centre stencil from the
 // cell above and below it and the shear-edge stencils around it.
-__global__ void __launch_bounds__(256) k_adj_vel(AdjArgs a)
+__device__ __forceinline__ void adj_vel_point(const AdjArgs &a, const int i, const int j, const int k)
 {
-    A_LOOP(a.b.nr + 1, a.b.nt + 1, a.b.np + 1)
     const Box &b = a.b;
     const Tables &t = a.t;
     const int nr = b.nr, nt = b.nt, np = b.np;
@@ -539,6 +535,23 @@ __global__ void __launch_bounds__(256) k_adj_vel(AdjArgs a)
     if (fT) a.out.f[L_VT][o] = vT;
     if (fP) a.out.f[L_VP][o] = vP;
     if (cen) a.out.f[L_E][o] = vE;
+}
+
+// One r-plane of a 64 x 4 (phi, th) tile per workgroup, planes slowest: the
+// workgroups of a few consecutive planes run together, so the i-1 / i+1
+// planes the stencils read are mostly served by the last-level cache.
+// (Walking several planes per workgroup, or dealing a tile's planes to one
+// XCD, measured slower at 511^3.)
+__global__ void __launch_bounds__(256) k_adj_sig(AdjArgs a)
+{
+    A_LOOP(a.b.nr, a.b.nt, a.b.np)
+    adj_sig_point(a, i, j, k);
+}
+
+__global__ void __launch_bounds__(256) k_adj_vel(AdjArgs a)
+{
+    A_LOOP(a.b.nr + 1, a.b.nt + 1, a.b.np + 1)
+    adj_vel_point(a, i, j, k);
 }
 
 // gradient with respect to rho: compute_V!'s -rho g on VR faces
