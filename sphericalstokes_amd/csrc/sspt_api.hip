@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -90,11 +91,31 @@ bool is_pingpong(int f)
 }
 
 // Allocates a buffer set; with `share`, non-ping-pong fields alias it.
+// The current set allocates its fields; the ping-pong set shares the
+// non-state fields.  With pair_state, a state field's two buffers are the
+// halves of one allocation: the current set allocates both, the ping-pong
+// set takes the upper half.
 int alloc_set(ss_ctx *c, FieldSet &s, const FieldSet *share)
 {
     for (int f = 0; f < SS_NFIELDS; ++f) {
         if (share && !is_pingpong(f)) {
             s.f[f] = share->f[f];
+            continue;
+        }
+        if (c->pair_state && is_pingpong(f)) {
+            if (share) {
+                s.f[f] = c->pair_base[f] + c->box_elems;
+                continue;
+            }
+            void *p = nullptr;
+            if (hipMalloc(&p, 2 * c->box_elems * sizeof(double)) != hipSuccess) {
+                c->err = "hipMalloc of a field box pair failed";
+                return SS_ERR_NOMEM;
+            }
+            c->pair_base[f] = (double *)p;
+            s.f[f] = (double *)p;
+            hipError_t e = hipMemsetAsync(p, 0, 2 * c->box_elems * sizeof(double), c->stream);
+            if (e != hipSuccess) return hip_fail(c, e, "hipMemsetAsync");
             continue;
         }
         void *p = nullptr;
@@ -110,10 +131,10 @@ int alloc_set(ss_ctx *c, FieldSet &s, const FieldSet *share)
     return SS_OK;
 }
 
-void free_set(FieldSet &s, const FieldSet *share)
+void free_set(ss_ctx *c, FieldSet &s, const FieldSet *share)
 {
     for (int f = 0; f < SS_NFIELDS; ++f) {
-        if (s.f[f] && !(share && s.f[f] == share->f[f])) (void)hipFree(s.f[f]);
+        if (s.f[f] && !c->pair_base[f] && !(share && s.f[f] == share->f[f])) (void)hipFree(s.f[f]);
         s.f[f] = nullptr;
     }
 }
@@ -158,6 +179,7 @@ int ss_create(int nr, int nth, int nph, int device, ss_ctx **out)
     c->box_elems = (size_t)(nr + 1) * (size_t)b.si;
     // the fused kernels address a field with 32-bit byte offsets
     if (c->box_elems * sizeof(double) > 0xFFFFFFFFull) { delete c; return SS_ERR_ARG; }
+    if (const char *ev = std::getenv("SSPT_PAIRED")) c->pair_state = std::atoi(ev) != 0;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return (int)e; }
     int rc = alloc_set(c, c->cur, nullptr);
@@ -197,8 +219,10 @@ int ss_destroy(ss_ctx *c)
     if (!c) return SS_ERR_ARG;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->alt_ready) free_set(c->alt, &c->cur);
-    free_set(c->cur, nullptr);
+    if (c->alt_ready) free_set(c, c->alt, &c->cur);
+    free_set(c, c->cur, nullptr);
+    for (double *&p : c->pair_base)
+        if (p) { (void)hipFree(p); p = nullptr; }
     if (c->staging) (void)hipFree(c->staging);
     if (c->red) (void)hipFree(c->red);
     if (c->red_host) (void)hipHostFree(c->red_host);
@@ -207,7 +231,9 @@ int ss_destroy(ss_ctx *c)
     if (c->ftab32_mem) (void)hipFree(c->ftab32_mem);
     for (double *p : c->tau_scr)
         if (p) (void)hipFree(p);
-    for (float *p : c->f32)
+    for (int q = 0; q < SS_F32_NBOX; ++q) // the correction pairs are freed through their lower halves
+        if (c->f32[q] && !(q < 20 && c->pair_base32[q % 10])) (void)hipFree(c->f32[q]);
+    for (float *p : c->pair_base32)
         if (p) (void)hipFree(p);
     for (double *p : c->adj)
         if (p) (void)hipFree(p);
@@ -542,7 +568,7 @@ int ensure_alt(ss_ctx *c)
 {
     if (c->alt_ready) return SS_OK;
     int rc = alloc_set(c, c->alt, &c->cur);
-    if (rc != SS_OK) { free_set(c->alt, &c->cur); return rc; }
+    if (rc != SS_OK) { free_set(c, c->alt, &c->cur); return rc; }
     c->alt_ready = true;
     return sync(c, "alloc ping-pong");
 }

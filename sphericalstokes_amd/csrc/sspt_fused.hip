@@ -219,6 +219,7 @@ static int pick_rchunk(int nr, int tiles_xy)
 template <typename R>
 void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_main1)
 {
+    a.pp_in = a.pp_out = a.ppe_in = a.ppe_out = 0;
     const int nx = (a.b.np + OK_ - 1) / OK_;
     const int ny = (a.b.nt + OJ<R> - 1) / OJ<R>;
     a.rchunk = pick_rchunk(a.b.nr, nx * ny);
@@ -228,18 +229,22 @@ void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_
     const int grid = ((T + 7) / 8) * 8;
     if (ev_main0) (void)hipEventRecord(ev_main0, c->stream);
     const dim3 G(grid), B(TK, TJB<R>);
+    Args<R> am = a; // main-kernel view; the boundary pass keeps the plain pointers
+    if (c->pair_state) pair_args(am, (long long)(c->box_elems * sizeof(R)));
     if constexpr (sizeof(R) == 8) {
-        switch (a.flags) {
-#define SSPT_FL(v) case v: hipLaunchKernelGGL((k_fused_main<R, v>), G, B, 0, c->stream, a, nx, ny, nz); break;
+        switch (am.flags) {
+#define SSPT_FL(v) case v: hipLaunchKernelGGL((k_fused_main<R, v>), G, B, 0, c->stream, am, nx, ny, nz); break;
         SSPT_FL(0) SSPT_FL(1) SSPT_FL(3) SSPT_FL(4) SSPT_FL(5) SSPT_FL(7)
         SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15)
+        SSPT_FL(32) SSPT_FL(33) SSPT_FL(35) SSPT_FL(36) SSPT_FL(37) SSPT_FL(39)
+        SSPT_FL(40) SSPT_FL(41) SSPT_FL(43) SSPT_FL(44) SSPT_FL(45) SSPT_FL(47)
 #undef SSPT_FL
         default: break; // CHECK without LAST is never requested in FP64
         }
     } else {
-        switch (a.flags) {
-#define SSPT_FL(v) case v: hipLaunchKernelGGL((k_fused_main<R, v>), G, B, 0, c->stream, a, nx, ny, nz); break;
-        SSPT_FL(16) SSPT_FL(18)
+        switch (am.flags) {
+#define SSPT_FL(v) case v: hipLaunchKernelGGL((k_fused_main<R, v>), G, B, 0, c->stream, am, nx, ny, nz); break;
+        SSPT_FL(16) SSPT_FL(18) SSPT_FL(48) SSPT_FL(50)
 #undef SSPT_FL
         default: break; // FP32 runs correction sweeps only
         }

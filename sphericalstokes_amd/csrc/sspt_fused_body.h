@@ -53,7 +53,9 @@ constexpr double PI_ = 3.14159265358979323846;
 // F_CORR: correction sweep of the FP32/FP64 defect-correction solve (forcing
 // arrays instead of rho*g and the continuity right-hand side, frozen eta, no
 // power law; homogeneous BCs come from sh = 0).
-enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP = 8, F_CORR = 16 };
+// F_PAIRED: the in and out buffers of each state field are the two halves of
+// one allocation (one field base serves both, see fat_pp / wout).
+enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP = 8, F_CORR = 16, F_PAIRED = 32 };
 
 struct BoxG {
     int nr, nt, np;
@@ -74,6 +76,14 @@ template <typename T> SSPT_DH inline T &fat(T *p, uint32_t o)
 {
     return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + (uint32_t)(o * (uint32_t)sizeof(T)));
 }
+// Element o of a field at byte offset `off` from base p (the in or out half
+// of a paired allocation; off + 8 o stays below 4 GiB).
+template <typename T> SSPT_DH inline const T &fat_pp(const T *p, uint32_t o, uint32_t off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint32_t)(o * (uint32_t)sizeof(T) + off));
+}
+// the in half of a state field (offset first, for reading at call sites)
+template <typename T> SSPT_DH inline const T &fat_in(const T *p, uint32_t off, uint32_t o) { return fat_pp(p, o, off); }
 template <typename T> SSPT_DH inline const T &fat(const T *p, uint32_t o)
 {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint32_t)(o * (uint32_t)sizeof(T)));
@@ -121,7 +131,57 @@ template <typename R> struct Args {
     int bc_mode;
     R r, sh;
     int rchunk; // r-steps per block along grid z
+    // byte offsets of the in and out halves of the state fields (viscosity
+    // apart: it is frozen in a correction sweep); 0 unless F_PAIRED, where
+    // the state pointers above are the lower halves
+    uint32_t pp_in, pp_out, ppe_in, ppe_out;
 };
+
+// Arguments of the main kernel for paired state buffers: when the in and
+// out buffers of every state field are the two halves of one allocation of
+// 2 x box bytes (below 4 GiB), the lower half becomes the field's only base
+// and the halves are addressed by byte offsets (F_PAIRED); this frees the
+// scalar registers of eleven out pointers.  FS = bytes of one box.  Returns
+// false (and changes nothing) when the buffers are not paired so.
+template <typename R> SSPT_DH inline bool pair_args(Args<R> &a, long long FS)
+{
+    if (2 * FS > 0xFFFFFFFFll) return false;
+    const R **in[10] = {&a.P, &a.tRR, &a.tTT, &a.tPP, &a.tRT, &a.tRP, &a.tTP, &a.VR, &a.VT, &a.VP};
+    R **out[10] = {&a.oP, &a.oRR, &a.oTT, &a.oPP, &a.oRT, &a.oRP, &a.oTP, &a.oVR, &a.oVT, &a.oVP};
+    const long long d0 = (const char *)*out[0] - (const char *)*in[0];
+    if (d0 != FS && d0 != -FS) return false;
+    for (int q = 1; q < 10; ++q)
+        if ((const char *)*out[q] - (const char *)*in[q] != d0) return false;
+    const bool corr = (a.flags & F_CORR) != 0;
+    if (!corr && (const char *)a.oEta - (const char *)a.eta != d0) return false;
+    const uint32_t pin = d0 > 0 ? 0u : (uint32_t)FS, pout = d0 > 0 ? (uint32_t)FS : 0u;
+    for (int q = 0; q < 10; ++q) {
+        const R *lo = d0 > 0 ? *in[q] : *out[q];
+        *in[q] = lo;
+        *out[q] = const_cast<R *>(lo);
+    }
+    a.pp_in = pin;
+    a.pp_out = pout;
+    if (!corr) {
+        const R *lo = d0 > 0 ? a.eta : a.oEta;
+        a.eta = lo;
+        a.oEta = const_cast<R *>(lo);
+        a.ppe_in = pin;
+        a.ppe_out = pout;
+    }
+    a.flags |= F_PAIRED;
+    return true;
+}
+
+// Destination of a state write: with F_PAIRED the out half of the pair
+// based at `in`, otherwise the separate out buffer.
+template <typename R, int FL> SSPT_DH inline R &wout(R *out, const R *in, uint32_t o, uint32_t pp)
+{
+    if constexpr ((FL & F_PAIRED) != 0)
+        return const_cast<R &>(fat_pp(in, o, pp));
+    else
+        return fat(out, o);
+}
 
 template <typename R> struct Lds {
     R VR[TJB<R>][TK], VT[TJB<R>][TK], VP[TJB<R>][TK];
@@ -243,10 +303,10 @@ template <typename R> SSPT_DH inline void load_ring(Thread<R> &T, const Args<R> 
     const BoxG &b = a.b;
     const int ie = clampi(p, 0, b.nr - 1);
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
-    if (T.ty == 0) T.ringJ = fat(a.eta, b.o32(ie, clampi(T.j - 1, 0, b.nt - 1), kcc));
-    if (T.ty == TJB<R> - 1) T.ringJ = fat(a.eta, b.o32(ie, clampi(T.j + 1, 0, b.nt - 1), kcc));
-    if (T.tx == 0) T.ringK = fat(a.eta, b.o32(ie, jcc, clampi(T.k - 1, 0, b.np - 1)));
-    if (T.tx == TK - 1) T.ringK = fat(a.eta, b.o32(ie, jcc, clampi(T.k + 1, 0, b.np - 1)));
+    if (T.ty == 0) T.ringJ = fat_in(a.eta, a.ppe_in, b.o32(ie, clampi(T.j - 1, 0, b.nt - 1), kcc));
+    if (T.ty == TJB<R> - 1) T.ringJ = fat_in(a.eta, a.ppe_in, b.o32(ie, clampi(T.j + 1, 0, b.nt - 1), kcc));
+    if (T.tx == 0) T.ringK = fat_in(a.eta, a.ppe_in, b.o32(ie, jcc, clampi(T.k - 1, 0, b.np - 1)));
+    if (T.tx == TK - 1) T.ringK = fat_in(a.eta, a.ppe_in, b.o32(ie, jcc, clampi(T.k + 1, 0, b.np - 1)));
 }
 
 template <typename R> SSPT_DH inline uint32_t own_centre(const Thread<R> &T, const BoxG &b, int s)
@@ -259,10 +319,10 @@ template <typename R> SSPT_DH inline uint32_t own_centre(const Thread<R> &T, con
 template <typename R, int FL> SSPT_DH inline void load_centre_old(Thread<R> &T, const Args<R> &a, int s)
 {
     const uint32_t oc = own_centre(T, a.b, s);
-    T.oP = fat(a.P, oc);
-    T.oRR = fat(a.tRR, oc);
-    T.oTT = fat(a.tTT, oc);
-    T.oPP = fat(a.tPP, oc);
+    T.oP = fat_in(a.P, a.pp_in, oc);
+    T.oRR = fat_in(a.tRR, a.pp_in, oc);
+    T.oTT = fat_in(a.tTT, a.pp_in, oc);
+    T.oPP = fat_in(a.tPP, a.pp_in, oc);
     if (FL & F_CORR) {
         T.ogP = fat(a.gP, oc);
         T.ofR = fat(a.fVR, oc);
@@ -271,13 +331,13 @@ template <typename R, int FL> SSPT_DH inline void load_centre_old(Thread<R> &T, 
     } else {
         T.orho = fat(a.rho, oc);
     }
-    T.oTPold = fat(a.tTP, a.b.o32(clampi(s, 0, a.b.nr - 1), T.jc, T.kc));
+    T.oTPold = fat_in(a.tTP, a.pp_in, a.b.o32(clampi(s, 0, a.b.nr - 1), T.jc, T.kc));
 }
 template <typename R> SSPT_DH inline void load_face_old(Thread<R> &T, const Args<R> &a, int s)
 {
     const uint32_t oe = a.b.o32(clampi(s + 1, 0, a.b.nr), T.jc, T.kc);
-    T.oRTold = fat(a.tRT, oe);
-    T.oRPold = fat(a.tRP, oe);
+    T.oRTold = fat_in(a.tRT, a.pp_in, oe);
+    T.oRPold = fat_in(a.tRP, a.pp_in, oe);
 }
 template <typename R, int FL> SSPT_DH inline void load_visc_old(Thread<R> &T, const Args<R> &a, int s)
 {
@@ -294,15 +354,15 @@ template <typename R, int FL> SSPT_DH inline void thread_prologue(Thread<R> &T, 
 {
     const BoxG &b = a.b;
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
-    T.e0 = fat(a.eta, b.o32(clampi(start, 0, b.nr - 1), jcc, kcc));
-    T.e1 = fat(a.eta, b.o32(clampi(start + 1, 0, b.nr - 1), jcc, kcc));
-    T.ne = fat(a.eta, b.o32(clampi(start + 2, 0, b.nr - 1), jcc, kcc));
-    T.vr1 = fat(a.VR, b.o32(clampi(start, 0, b.nr), T.jc, T.kc));
-    T.vt1 = fat(a.VT, b.o32(clampi(start, 0, b.nr - 1), T.jc, T.kc));
-    T.vp1 = fat(a.VP, b.o32(clampi(start, 0, b.nr - 1), T.jc, T.kc));
-    T.nvr = fat(a.VR, b.o32(clampi(start + 1, 0, b.nr), T.jc, T.kc));
-    T.nvt = fat(a.VT, b.o32(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc));
-    T.nvp = fat(a.VP, b.o32(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc));
+    T.e0 = fat_in(a.eta, a.ppe_in, b.o32(clampi(start, 0, b.nr - 1), jcc, kcc));
+    T.e1 = fat_in(a.eta, a.ppe_in, b.o32(clampi(start + 1, 0, b.nr - 1), jcc, kcc));
+    T.ne = fat_in(a.eta, a.ppe_in, b.o32(clampi(start + 2, 0, b.nr - 1), jcc, kcc));
+    T.vr1 = fat_in(a.VR, a.pp_in, b.o32(clampi(start, 0, b.nr), T.jc, T.kc));
+    T.vt1 = fat_in(a.VT, a.pp_in, b.o32(clampi(start, 0, b.nr - 1), T.jc, T.kc));
+    T.vp1 = fat_in(a.VP, a.pp_in, b.o32(clampi(start, 0, b.nr - 1), T.jc, T.kc));
+    T.nvr = fat_in(a.VR, a.pp_in, b.o32(clampi(start + 1, 0, b.nr), T.jc, T.kc));
+    T.nvt = fat_in(a.VT, a.pp_in, b.o32(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc));
+    T.nvp = fat_in(a.VP, a.pp_in, b.o32(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc));
     T.ringJ = T.ringK = R(0);
     load_ring(T, a, start + 1);
     load_centre_old<R, FL>(T, a, start);
@@ -336,10 +396,10 @@ SSPT_DH inline void phase1(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s)
     if (T.tx == TK - 1) L.ETA[T.ty + 1][TK + 1] = T.ringK;
     // prefetch for step s+1
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
-    T.nvr = fat(a.VR, b.o32(clampi(s + 2, 0, nr), T.jc, T.kc));
-    T.nvt = fat(a.VT, b.o32(clampi(s + 2, 0, nr - 1), T.jc, T.kc));
-    T.nvp = fat(a.VP, b.o32(clampi(s + 2, 0, nr - 1), T.jc, T.kc));
-    T.ne = fat(a.eta, b.o32(clampi(s + 3, 0, nr - 1), jcc, kcc));
+    T.nvr = fat_in(a.VR, a.pp_in, b.o32(clampi(s + 2, 0, nr), T.jc, T.kc));
+    T.nvt = fat_in(a.VT, a.pp_in, b.o32(clampi(s + 2, 0, nr - 1), T.jc, T.kc));
+    T.nvp = fat_in(a.VP, a.pp_in, b.o32(clampi(s + 2, 0, nr - 1), T.jc, T.kc));
+    T.ne = fat_in(a.eta, a.ppe_in, b.o32(clampi(s + 3, 0, nr - 1), jcc, kcc));
     load_ring(T, a, s + 2);
 }
 
@@ -371,7 +431,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         } else {
             const int dty = cj - T.j, dtx = ck - T.k;
             const int ty = clampi(T.ty + dty, 0, TJB<R> - 1), tx = clampi(T.tx + dtx, 0, TK - 1);
-            const R em = fat(a.eta, b.o32(p - 1, cj, ck)), ep = fat(a.eta, b.o32(p + 1, cj, ck));
+            const R em = fat_in(a.eta, a.ppe_in, b.o32(p - 1, cj, ck)), ep = fat_in(a.eta, a.ppe_in, b.o32(p + 1, cj, ck));
             T.sm1 = star(L, ty, tx, em, ep);
         }
     } else if (p == nr - 1) {
@@ -409,10 +469,10 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         T.nSPP = -nP + T.nPP;
         if (wr && T.own) {
             const uint32_t o = b.o32(s, T.j, T.k);
-            fat(a.oP, o) = nP;
-            fat(a.oRR, o) = T.nRR;
-            fat(a.oTT, o) = T.nTT;
-            fat(a.oPP, o) = T.nPP;
+            wout<R, FL>(a.oP, a.P, o, a.pp_out) = nP;
+            wout<R, FL>(a.oRR, a.tRR, o, a.pp_out) = T.nRR;
+            wout<R, FL>(a.oTT, a.tTT, o, a.pp_out) = T.nTT;
+            wout<R, FL>(a.oPP, a.tPP, o, a.pp_out) = T.nPP;
             if ((FL & F_LAST) && !(FL & F_CORR)) {
                 fat(a.DT, o) = T.nDT;
                 fat(a.SM, o) = T.sm0;
@@ -436,7 +496,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         const R D = ((tload(t.tr, s * NTR + TR_invRc4) * dVP) * a._dth + (tload(t.t2, (s * (nt + 1) + T.j) * NT2 + T2_c4) * dVT) * a._dph) -
                          div_rcp(avVP, tload(t.tr, s * NTR + TR_Rc4), tload(t.tr, s * NTR + TR_invRc4)) * tload(t.tt, T.j * NTT + TT_cotf4);
         T.ntp = (T.oTPold + T.gtp * D) / (R(1) + T.gtp / T.etp);
-        if (wr && T.own) fat(a.oTP, b.o32(s, T.j, T.k)) = T.ntp;
+        if (wr && T.own) wout<R, FL>(a.oTP, a.tTP, b.o32(s, T.j, T.k), a.pp_out) = T.ntp;
     }
     L.TP[T.ty][T.tx] = T.ntp;
 
@@ -462,7 +522,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         const R dt = (T.cDT + T.nDT) * R(0.5);
         const R v = T.vr0 + dV * dt;
         const uint32_t o = b.o32(s, T.j, T.k);
-        fat(a.oVR, o) = v;
+        wout<R, FL>(a.oVR, a.VR, o, a.pp_out) = v;
         if (FL & F_CHECK) {
             const R d = v - T.vr0;
             if (!(FL & F_CORR)) fat(a.VRe, o) = d;
@@ -516,7 +576,7 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R D = ((T.vt1 - T.vt0) * a._dr + (iRe * (T.vr1 - L.VR[T.ty - 1][T.tx])) * a._dth) -
                              div_rcp((T.vt0 + T.vt1) * R(0.5), Re, iRe);
             T.nrt = (T.oRTold + G * D) / (R(1) + G / E);
-            if (wr && T.own) fat(a.oRT, b.o32(f, T.j, T.k)) = T.nrt;
+            if (wr && T.own) wout<R, FL>(a.oRT, a.tRT, b.o32(f, T.j, T.k), a.pp_out) = T.nrt;
         }
         if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
             const R G = ((T.pG_rp + gs * L.SM[T.ty][T.tx - 1]) + gs * T.sm1) * R(0.25);
@@ -524,7 +584,7 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R D = ((T.vp1 - T.vp0) * a._dr + (tload(t.t2, (f * (nt + 1) + T.j) * NT2 + T2_e) * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
                              div_rcp((T.vp0 + T.vp1) * R(0.5), Re, iRe);
             T.nrp = (T.oRPold + G * D) / (R(1) + G / E);
-            if (wr && T.own) fat(a.oRP, b.o32(f, T.j, T.k)) = T.nrp;
+            if (wr && T.own) wout<R, FL>(a.oRP, a.tRP, b.o32(f, T.j, T.k), a.pp_out) = T.nrp;
         }
     }
     L.RT[T.ty][T.tx] = T.nrt;
@@ -555,7 +615,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * R(0.5);
             const R v = T.vt0 + dV * dt;
             const uint32_t o = b.o32(s, T.j, T.k);
-            fat(a.oVT, o) = v;
+            wout<R, FL>(a.oVT, a.VT, o, a.pp_out) = v;
             if (FL & F_CHECK) {
                 const R d = v - T.vt0;
                 if (!(FL & F_CORR)) fat(a.VTe, o) = d;
@@ -580,7 +640,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * R(0.5);
             const R v = T.vp0 + dV * dt;
             const uint32_t o = b.o32(s, T.j, T.k);
-            fat(a.oVP, o) = v;
+            wout<R, FL>(a.oVP, a.VP, o, a.pp_out) = v;
             if (FL & F_CHECK) {
                 const R d = v - T.vp0;
                 if (!(FL & F_CORR)) fat(a.VPe, o) = d;
@@ -601,7 +661,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             R pl;
             const R e = power_law_pt(tii, T.oini, T.oPL, &pl, a, FL);
             const uint32_t o = b.o32(s, T.j, T.k);
-            fat(a.oEta, o) = e;
+            wout<R, FL>(a.oEta, a.eta, o, a.ppe_out) = e;
             if (FL & F_RELAX) fat(a.oPL, o) = pl;
             if (FL & F_LAST) fat(a.TII, o) = tii;
         }

@@ -85,6 +85,9 @@ struct ss_ctx {
     sspt::FieldSet cur{};   // current state
     sspt::FieldSet alt{};   // ping-pong partners used by the fused loop
     bool alt_ready = false;
+    bool pair_state = true;         // allocate in/out halves of state fields together (fused kernel F_PAIRED)
+    double *pair_base[SS_NFIELDS] = {}; // lower halves of the paired FP64 state fields
+    float *pair_base32[10] = {};        // FP32 correction pairs (set A | set B)
     double *tab_mem = nullptr;
     sspt::Tables tab{};
     double *ftab_mem = nullptr;     // interleaved tables of the fused kernel

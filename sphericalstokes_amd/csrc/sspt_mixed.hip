@@ -299,7 +299,15 @@ int ensure_f32(ss_ctx *c)
     if (c->f32_ready) return SS_OK;
     const size_t n = c->box_elems;
     for (int q = 0; q < SS_F32_NBOX; ++q) {
-        if (hipMalloc(&c->f32[q], n * sizeof(float)) != hipSuccess) return mfail(c, SS_ERR_NOMEM, "FP32 boxes");
+        if (c->pair_state && q >= 10 && q < 20) {
+            c->f32[q] = c->pair_base32[q - 10] + n; // upper half of the pair (set A | set B)
+        } else if (c->pair_state && q < 10) {
+            if (hipMalloc(&c->pair_base32[q], 2 * n * sizeof(float)) != hipSuccess)
+                return mfail(c, SS_ERR_NOMEM, "FP32 boxes");
+            c->f32[q] = c->pair_base32[q];
+        } else if (hipMalloc(&c->f32[q], n * sizeof(float)) != hipSuccess) {
+            return mfail(c, SS_ERR_NOMEM, "FP32 boxes");
+        }
         hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, c->stream, c->f32[q], n);
     }
     c->f32_ready = true;

@@ -27,7 +27,7 @@ def lib():
         L.emu_fused_iterations.restype = C.c_int
         L.emu_fused_iterations.argtypes = [C.c_int] * 4 + [DP] * 3 + [C.POINTER(ss_params), C.POINTER(DP),
                                            C.POINTER(DP), C.c_int, C.c_int, C.c_int,
-                                           C.POINTER(C.c_ulonglong)]
+                                           C.POINTER(C.c_ulonglong), C.c_longlong]
         _L = L
     return _L
 
@@ -58,11 +58,22 @@ def from_box(setup, box, shapes):
     return out
 
 
-def run(setup, fields, params, niter, check_last=False, rchunk=0):
-    """Runs the fused schedule on the host; returns (fields, red[6])."""
+def run(setup, fields, params, niter, check_last=False, rchunk=0, paired=False):
+    """Runs the fused schedule on the host; returns (fields, red[6]).  With
+    paired, each state field's two buffers are the halves of one array and
+    the main kernel runs with F_PAIRED."""
     L = lib()
     cur = to_box(setup, fields)
-    alt = {n: (np.zeros_like(cur[n]) if n in PINGPONG else cur[n]) for n in FIELDS}
+    if paired:
+        pairs = {n: np.zeros((2,) + cur[n].shape) for n in PINGPONG}
+        for n in PINGPONG:
+            pairs[n][0] = cur[n]
+        alt = {n: (pairs[n][1] if n in PINGPONG else cur[n]) for n in FIELDS}
+        cur = {n: (pairs[n][0] if n in PINGPONG else cur[n]) for n in FIELDS}
+        pair_bytes = cur["P"].nbytes
+    else:
+        alt = {n: (np.zeros_like(cur[n]) if n in PINGPONG else cur[n]) for n in FIELDS}
+        pair_bytes = 0
     DP = C.POINTER(C.c_double)
     pc = (DP * len(FIELDS))(*[cur[n].ctypes.data_as(DP) for n in FIELDS])
     pa = (DP * len(FIELDS))(*[alt[n].ctypes.data_as(DP) for n in FIELDS])
@@ -71,7 +82,9 @@ def run(setup, fields, params, niter, check_last=False, rchunk=0):
     R, th, ph = (np.ascontiguousarray(x, dtype=np.float64) for x in (setup.R, setup.th, setup.ph))
     which = L.emu_fused_iterations(setup.nr, setup.nth, setup.nph, pitch_of(setup.nph), R.ctypes.data_as(DP),
                                    th.ctypes.data_as(DP), ph.ctypes.data_as(DP), C.byref(sp), pc, pa,
-                                   int(niter), int(check_last), int(rchunk), red)
+                                   int(niter), int(check_last), int(rchunk), red, pair_bytes)
+    if which < 0:
+        raise RuntimeError("state buffers are not paired")
     box = cur if which == 0 else alt
     shapes = {n: fields[n].shape for n in FIELDS}
     reds = np.array(list(red), dtype=np.uint64).view(np.float64)

@@ -70,6 +70,8 @@ template <typename R> void run_main(const Args<R> &a, int rchunk, unsigned long 
 #define SSPT_FL(v) case v: run_main_t<R, v>(a, rchunk, red); break;
     SSPT_FL(0) SSPT_FL(1) SSPT_FL(3) SSPT_FL(4) SSPT_FL(5) SSPT_FL(7)
     SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15) SSPT_FL(16) SSPT_FL(18)
+    SSPT_FL(32) SSPT_FL(33) SSPT_FL(35) SSPT_FL(36) SSPT_FL(37) SSPT_FL(39)
+    SSPT_FL(40) SSPT_FL(41) SSPT_FL(43) SSPT_FL(44) SSPT_FL(45) SSPT_FL(47)
 #undef SSPT_FL
     default: break;
     }
@@ -123,10 +125,13 @@ extern "C" {
 // must alias between the two sets.  The last iteration writes the auxiliary
 // fields; with check_last it is also a convergence-check iteration whose
 // max-reductions land in red[6].  Returns 0 if the final state is in `cur`,
-// 1 if it is in `alt`.
+// 1 if it is in `alt`.  pair_bytes > 0: the state buffers of cur and alt are
+// halves of paired allocations pair_bytes apart and the main kernel runs
+// with F_PAIRED (as the library does); the boundary pass keeps the plain
+// pointers.
 int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, const double *th1, const double *ph1,
                          const ss_params *p, double **cur, double **alt, int niter, int check_last, int rchunk,
-                         unsigned long long red[6])
+                         unsigned long long red[6], long long pair_bytes)
 {
     HostBox hb{nr, nt, np, pitch, (long long)(nt + 1) * pitch};
     std::vector<double> tab, ftab;
@@ -169,7 +174,9 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
                   ((p->eta_min > 0.0 || p->eta_max < INFINITY) ? F_CLAMP : 0);
         a.bc_mode = p->bc_mode; a.r = p->r; a.sh = p->sh;
         a.rchunk = rchunk > 0 ? rchunk : nr;
-        run_main(a, a.rchunk, red);
+        Args<double> am = a;
+        if (pair_bytes > 0 && !pair_args(am, pair_bytes)) return -1;
+        run_main(am, a.rchunk, red);
         run_boundary(a, red);
         std::swap(in, out);
         which ^= 1;

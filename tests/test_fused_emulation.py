@@ -94,3 +94,17 @@ def test_powerlaw_fused_vs_emulated_chunks_bitwise():
     b, _ = emu.run(s, s.fields, s.params, 3, rchunk=4)
     res, bad = compare_fields(b, a)
     assert not bad, {k: res[k] for k in bad}
+
+
+@pytest.mark.parametrize("kind,kw", CASES)
+@pytest.mark.parametrize("niter", [1, 2])
+def test_paired_buffers_give_the_same_iterates(kind, kw, niter):
+    """With each state field's in/out buffers the two halves of one array,
+    the main kernel addresses both through one base and byte offsets
+    (F_PAIRED); the iterates must not change by a bit (an odd number of
+    iterations leaves the state in the upper halves)."""
+    s = _case(kind, **kw)
+    a, _ = emu.run(s, s.fields, s.params, niter)
+    b, _ = emu.run(s, s.fields, s.params, niter, paired=True)
+    res, bad = compare_fields(b, a)
+    assert not bad, {k: res[k] for k in bad}
