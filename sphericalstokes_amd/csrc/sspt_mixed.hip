@@ -432,6 +432,12 @@ int ss_pt_solve_mixed(ss_ctx *c, const ss_params *p, const ss_mixed_opts *o, ss_
     a.vpdt2 = (float)(p->vpdt * p->vpdt);
     a.g = 0.f; a.tC = 1.f; a.ex = 0.f; a.relax = 1.f; a.emin = 0.f; a.emax = INFINITY;
     a.bc_mode = p->bc_mode; a.r = (float)p->r; a.sh = 0.f;
+    // The residual reduction of an outer step scales with the inner
+    // tolerance it used (the error left in the correction).  When the
+    // reduction still needed is smaller than the last step's, the inner
+    // tolerance is loosened in proportion (with a factor 2 of margin); it is
+    // never tightened beyond o->tol_inner.
+    double tin_prev = o->tol_inner, r_prev = NAN;
     for (int outer = 0; outer < o->max_outer; ++outer) {
         double r, rd, rm;
         // tau := consistent stress (in place), residual -> FP32 forcing
@@ -441,6 +447,14 @@ int ss_pt_solve_mixed(ss_ctx *c, const ss_params *p, const ss_mixed_opts *o, ss_
         res->r = r;
         if (std::isnan(r)) return mfail(c, SS_ERR_NONFINITE, "NaN residual");
         if (r <= o->tol) { res->converged = 1; break; }
+        double tin = o->tol_inner;
+        if (std::isfinite(r_prev) && r_prev > 0.0 && r < r_prev) {
+            const double q = r / r_prev, want = 0.5 * (o->tol / r);
+            if (want > q) tin = std::fmin(1e-2, tin_prev * (want / q));
+            tin = std::fmax(tin, o->tol_inner);
+        }
+        r_prev = r;
+        tin_prev = tin;
         // frozen FP32 viscosity; zero correction (both buffer sets)
         hipLaunchKernelGGL(k_to_f32, mgrid(b.nr, b.nt, b.np), MB, 0, c->stream, b, F[SS_F32_ETA], c->cur.f[SS_ETA], b.nr,
                            b.nt, b.np);
@@ -471,7 +485,7 @@ int ss_pt_solve_mixed(ss_ctx *c, const ss_params *p, const ss_mixed_opts *o, ss_
                     if (vv > 0.0) em = std::fmax(em, dv / vv);
                 }
                 err_in = em;
-                if (err_in < o->tol_inner) break;
+                if (err_in < tin) break;
             }
         }
         if ((rc = msync(c, "inner sweeps")) != SS_OK) return rc;
