@@ -187,6 +187,7 @@ int ss_create(int nr, int nth, int nph, int device, ss_ctx **out)
     c->staging_elems = (size_t)(nr + 1) * (nth + 1) * (nph + 1);
     if (hipMalloc(&c->staging, c->staging_elems * sizeof(double)) != hipSuccess) { ss_destroy(c); return SS_ERR_NOMEM; }
     if (hipMalloc(&c->red, 16 * sizeof(unsigned long long)) != hipSuccess) { ss_destroy(c); return SS_ERR_NOMEM; }
+    if (hipMalloc(&c->tbp, 2 * (size_t)nr * nth * sizeof(double)) != hipSuccess) { ss_destroy(c); return SS_ERR_NOMEM; }
     if (hipHostMalloc(&c->red_host, 16 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         ss_destroy(c);
         return SS_ERR_NOMEM;
@@ -225,6 +226,7 @@ int ss_destroy(ss_ctx *c)
         if (p) { (void)hipFree(p); p = nullptr; }
     if (c->staging) (void)hipFree(c->staging);
     if (c->red) (void)hipFree(c->red);
+    if (c->tbp) (void)hipFree(c->tbp);
     if (c->red_host) (void)hipHostFree(c->red_host);
     if (c->tab_mem) (void)hipFree(c->tab_mem);
     if (c->ftab_mem) (void)hipFree(c->ftab_mem);

@@ -122,8 +122,8 @@ __global__ void __launch_bounds__(256) k_fused_boundary(Args<R> a)
         }
     } else {
         if (u >= 1 && u < nt - 1 && v >= 1 && v < nr - 1) {
-            boundary_eta_point(a, v, u, 0);
-            boundary_eta_point(a, v, u, np - 1);
+            boundary_eta_point(a, v, u, 0, 0);
+            boundary_eta_point(a, v, u, np - 1, 1);
         }
     }
     if (a.flags & F_CHECK) flush_max(a, mdv, mv, (threadIdx.x + threadIdx.y * blockDim.x) & 63);
@@ -173,6 +173,7 @@ static Args<double> make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out
     a.SM = out.f[SS_ETASM];
     a.DT = out.f[SS_DT_R];
     a.TII = out.f[SS_TII];
+    a.TBP = c->tbp;
     a.VRe = out.f[SS_VR_ERR];
     a.VTe = out.f[SS_VTH_ERR];
     a.VPe = out.f[SS_VPH_ERR];

@@ -69,6 +69,13 @@ struct BoxG {
     }
 };
 
+// Element index in the phi-face tII buffer ([2][nr][nt], side 0 at k = 1,
+// side 1 at k = np-2).
+SSPT_DH inline uint32_t tbp_off(const BoxG &b, int side, int i, int j)
+{
+    return ((uint32_t)side * (uint32_t)b.nr + (uint32_t)i) * (uint32_t)b.nt + (uint32_t)j;
+}
+
 // Element o of a field through a 32-bit byte offset from the field's base:
 // with a wave-uniform base this is the SGPR-base + 32-bit VGPR-offset form of
 // global memory instructions, with no 64-bit address arithmetic per access.
@@ -122,6 +129,10 @@ template <typename R> struct Args {
     R *oVR, *oVT, *oVP, *oEta, *oP, *oRR, *oTT, *oPP, *oRT, *oRP, *oTP, *oPL;
     // auxiliary outputs (last iteration / convergence check)
     R *SM, *DT, *TII, *VRe, *VTe, *VPe;
+    // tII of the interior cells next to the phi faces, [side][i][j] with
+    // side 0 at k = 1 and side 1 at k = np-2: written by the main kernel,
+    // read by the boundary pass for the viscosity of the phi-face cells
+    R *TBP;
     // forcing of a correction sweep (F_CORR): divV of the FP64 iterate at
     // centres and its momentum residual at faces
     const R *gP, *fVR, *fVT, *fVP;
@@ -664,6 +675,8 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             wout<R, FL>(a.oEta, a.eta, o, a.ppe_out) = e;
             if (FL & F_RELAX) fat(a.oPL, o) = pl;
             if (FL & F_LAST) fat(a.TII, o) = tii;
+            if (T.k == 1) fat(a.TBP, tbp_off(b, 0, s, T.j)) = tii;
+            if (T.k == np - 2) fat(a.TBP, tbp_off(b, 1, s, T.j)) = tii;
         }
     }
     // carries for step s+1
@@ -805,11 +818,15 @@ template <typename R> SSPT_DH inline void ghost_v_point(const Args<R> &a, int co
     }
 }
 
-template <typename R> SSPT_DH inline void boundary_eta_point(const Args<R> &a, int i, int j, int k)
+// Viscosity of boundary cell (i,j,k) from tII at the clamped interior cell
+// (copy_BC!(tII) then power_law!).  phi_side >= 0: (i,j,k) lies on that phi
+// face with 1 <= i <= nr-2, 1 <= j <= nt-2, and its clamped cell's tII was
+// stored by the main kernel; otherwise it is evaluated from the stresses.
+template <typename R> SSPT_DH inline void boundary_eta_point(const Args<R> &a, int i, int j, int k, int phi_side = -1)
 {
     const BoxG &b = a.b;
     const int ci = clampi(i, 1, b.nr - 2), cj = clampi(j, 1, b.nt - 2), ck = clampi(k, 1, b.np - 2);
-    const R tii = tii_inner(a, ci, cj, ck);
+    const R tii = phi_side >= 0 ? fat(a.TBP, tbp_off(b, phi_side, i, j)) : tii_inner(a, ci, cj, ck);
     const long long o = b.off(i, j, k);
     R pl;
     const R oldpl = (a.flags & F_RELAX) ? a.ePL[o] : R(0);

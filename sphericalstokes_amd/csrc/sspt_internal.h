@@ -105,6 +105,7 @@ struct ss_ctx {
     double *staging = nullptr;      // upload/download staging (largest field)
     size_t staging_elems = 0;
     unsigned long long *red = nullptr; // reduction slots (device)
+    double *tbp = nullptr; // tII next to the phi faces, [2][nr][nth] (fused iteration)
     unsigned long long *red_host = nullptr; // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> ev_pool;       // per-launch timing events

@@ -106,8 +106,8 @@ template <typename R> void run_boundary(const Args<R> &a, unsigned long long red
                     boundary_eta_point(a, v, 0, u);
                     boundary_eta_point(a, v, nt - 1, u);
                 } else if (fam == 2 && u >= 1 && u < nt - 1 && v >= 1 && v < nr - 1) {
-                    boundary_eta_point(a, v, u, 0);
-                    boundary_eta_point(a, v, u, np - 1);
+                    boundary_eta_point(a, v, u, 0, 0);
+                    boundary_eta_point(a, v, u, np - 1, 1);
                 }
                 for (int c = 0; c < 3; ++c) {
                     red[c] = std::max(red[c], mdv[c]);
@@ -148,6 +148,7 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
     static constexpr double PI = 3.14159265358979323846;
     for (int c = 0; c < 6; ++c) red[c] = 0ull;
     double **in = cur, **out = alt;
+    std::vector<double> tbp(2 * (size_t)nr * nt, std::nan(""));
     int which = 0;
     for (int it = 0; it < niter; ++it) {
         const bool last = it == niter - 1, check = last && check_last;
@@ -162,6 +163,7 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
         a.oP = out[SS_P]; a.oRR = out[SS_TAU_RR]; a.oTT = out[SS_TAU_THTH]; a.oPP = out[SS_TAU_PHPH];
         a.oRT = out[SS_TAU_RTH]; a.oRP = out[SS_TAU_RPH]; a.oTP = out[SS_TAU_THPH]; a.oPL = out[SS_ETA_PL];
         a.SM = out[SS_ETASM]; a.DT = out[SS_DT_R]; a.TII = out[SS_TII];
+        a.TBP = tbp.data();
         a.VRe = out[SS_VR_ERR]; a.VTe = out[SS_VTH_ERR]; a.VPe = out[SS_VPH_ERR];
         a.red = nullptr;
         a._dr = p->_dr; a._dth = p->_dth; a._dph = p->_dph;
