@@ -3,7 +3,7 @@
 ``build_all()`` compiles, in-tree:
   * sphericalstokes_amd/libsspt.so   -- the product: HIP kernels + C ABI (hipcc, gfx950)
   * oracle/libsspt_oracle.so         -- CPU restatement used as parity checker (gcc)
-  * tests/emu/libsspt_emu.so         -- host emulation of the fused kernel (g++)
+  * tests/emu/libsspt_emu.so         -- host emulation of the fused kernels (g++)
 
 FP contraction is off everywhere (``-ffp-contract=off``) so that the device
 arithmetic rounds like the reference expressions; packed FP32 instructions
@@ -22,7 +22,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libsspt.so"
 ORACLE_DIR = ROOT / "oracle"
-EMU_SRC = ROOT / "tests" / "emu" / "fused_emu.cpp"
+EMU_SRC = [ROOT / "tests" / "emu" / "fused_emu.cpp", ROOT / "tests" / "emu" / "adj_emu.cpp"]
 EMU_LIB = ROOT / "tests" / "emu" / "libsspt_emu.so"
 
 HIP_SOURCES = ["sspt_api.hip", "sspt_kernels.hip", "sspt_fused.hip", "sspt_mixed.hip", "sspt_adjoint.hip"]
@@ -72,7 +72,7 @@ def build_oracle() -> Path:
 def build_emu() -> Path:
     tmp = EMU_LIB.with_suffix(".so.tmp")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-          "-march=x86-64-v2", EMU_SRC, "-o", tmp])
+          "-march=x86-64-v2", *EMU_SRC, "-o", tmp])
     os.replace(tmp, EMU_LIB)
     return EMU_LIB
 

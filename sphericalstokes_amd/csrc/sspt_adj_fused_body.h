@@ -1,0 +1,589 @@
+// sspt_adj_fused_body.h -- one application of the transposed PT map
+// (lam_out = T^T lam_in at the converged state) as an r-marching block
+// schedule, the adjoint counterpart of sspt_fused_body.h.
+//
+// The forward iteration (SphericalStokes_diapir.jl:413-425) is, per point,
+// compute_P! (new P, tau from old V, eta, tau), compute_TII! + copy_BC!,
+// power_law! (new eta from tII) and compute_V! (new V from new P, tau).  Its
+// transpose, in gather form (each output sums the coefficients of the forward
+// statements that read it; no atomics), needs at plane q:
+//   * w_f = DT_f lam_V_f on the faces compute_V! updates (r-faces q, q+1;
+//     th/phi faces q-1, q);
+//   * Q = lam_tII / tII of interior cells (lam_tII from the power-law
+//     adjoint of the cell and of the boundary cells copy_BC! filled from it);
+//   * the adjoints of the new P, tau of centre q and of the edges stored at
+//     box index q ("sig", written scaled by compute_P!'s 1/den, which makes
+//     them the adjoints of the old P, tau);
+//   * lam_V' of faces and lam_eta' of centres from the sig values of the
+//     centres and edges around them ("vel").
+// Step q of a block computes the w's and Q of plane q (phase A), sig of
+// plane q (phase B) and the vel contributions of plane q's sig values
+// (phase C): VR(q) is complete then, while Vth, Vph and eta of plane q
+// still wait for the edges of box index q+1 and are finished at step q+1.
+// Values the same column needs at the next plane are carried in registers;
+// values neighbours need go through LDS (two barriers per step).
+//
+// Block = 64 lanes (phi) x ATJ rows (th); lane tx covers k = kb-1+tx, row ty
+// covers j = jb-1+ty; the inner 62 x (ATJ-2) positions are owned.  Outputs
+// cover the face extents: j in [0, nth], k in [0, nph], r-planes [0, nr].
+//
+// The BC fold (the transposes of BC_R!/BC_th!/BC_ph!) has been applied to
+// lam_in beforehand, and LB holds, for each interior cell next to the
+// boundary, the power-law adjoints of the boundary cells copy_BC! filled
+// from it.  The coefficients that depend only on the converged state are
+// recomputed from it (GS, KB, 1/den, 1/tII, the 4-point averages of the
+// viscosities and shear stresses).  The shear stresses of the plane ahead
+// are loaded one step early and carried.
+#pragma once
+#include "sspt_fused_body.h"
+
+namespace sspt {
+namespace adjf {
+
+using fused::BoxG;
+using fused::TabsG;
+using fused::TK;
+using fused::clampi;
+using fused::fat;
+using fused::tload;
+using fused::NTR;
+using fused::NTT;
+using fused::NT2;
+
+#ifndef SSPT_ATJB
+#define SSPT_ATJB 8
+#endif
+constexpr int ATJ = SSPT_ATJB;    // rows per block
+constexpr int AOK = TK - 2;       // owned lanes per row
+constexpr int AOJ = ATJ - 2;      // owned rows
+
+// adjoint state fields, in the order of the adjoint sets
+enum { A_VR, A_VT, A_VP, A_P, A_RR, A_TT, A_PP, A_RT, A_RP, A_TP, A_E, A_PL, A_N };
+
+struct AArgs {
+    BoxG b;
+    TabsG<double> t;
+    const double *li[A_N]; // lam in (BC-folded)
+    double *lo[A_N];       // lam out
+    // converged forward state
+    const double *tRR, *tTT, *tPP, *tRT, *tRP, *tTP, *eta, *ini, *ePL, *tII, *DT, *SM;
+    const double *LB;
+    // objective weights on the r-faces inside the inclusion (or null)
+    const double *bJ;
+    double obj_r, obj_rad;
+    double _dr, _dth, _dph, kb, gs, tC, ex, relax, emin, emax;
+    int do_relax, do_lo, do_hi;
+    int rchunk;
+};
+
+// ---------------------------------------------------------------- ranges
+// faces compute_V! updates and the edge boxes, in box indices
+SSPT_DH inline bool upd_VR(const BoxG &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 1 && j >= 1 && j <= b.nt - 2 && k >= 1 && k <= b.np - 2;
+}
+SSPT_DH inline bool upd_VT(const BoxG &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 2 && j >= 1 && j <= b.nt - 1 && k >= 1 && k <= b.np - 2;
+}
+SSPT_DH inline bool upd_VP(const BoxG &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 2 && j >= 1 && j <= b.nt - 2 && k >= 1 && k <= b.np - 1;
+}
+SSPT_DH inline bool in_RT(const BoxG &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 1 && j >= 1 && j <= b.nt - 1 && k >= 1 && k <= b.np - 2;
+}
+SSPT_DH inline bool in_RP(const BoxG &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 1 && j >= 1 && j <= b.nt - 2 && k >= 1 && k <= b.np - 1;
+}
+SSPT_DH inline bool in_TP(const BoxG &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 2 && j >= 1 && j <= b.nt - 1 && k >= 1 && k <= b.np - 1;
+}
+SSPT_DH inline bool inner_c(const BoxG &b, int i, int j, int k)
+{
+    return i >= 1 && i <= b.nr - 2 && j >= 1 && j <= b.nt - 2 && k >= 1 && k <= b.np - 2;
+}
+SSPT_DH inline bool is_cell(const BoxG &b, int i, int j, int k)
+{
+    return i >= 0 && i < b.nr && j >= 0 && j < b.nt && k >= 0 && k < b.np;
+}
+// an interior cell that copy_BC! copies onto a boundary cell
+SSPT_DH inline bool next_to_boundary(const BoxG &b, int i, int j, int k)
+{
+    return i == 1 || i == b.nr - 2 || j == 1 || j == b.nt - 2 || k == 1 || k == b.np - 2;
+}
+
+// power_law!'s adjoint at a cell from lam_eta (le), tII, eta_ini and, with
+// relaxation, the old eta_PL and lam_eta_PL: returns lam_tII; *pl_out = the
+// adjoint of the old eta_PL (relaxation only).
+SSPT_DH inline double adj_power_law_v(const AArgs &a, double le, double tii, double ini, double ePL, double lPL,
+                                      double *pl_out)
+{
+    const double pl0 = ini * std::pow(tii / a.tC, a.ex);
+    const double pl = a.do_relax ? ePL : pl0;
+    const double e_raw = 2.0 / (1.0 / ini + 1.0 / pl);
+    const bool clamped = (a.do_lo && e_raw < a.emin) || (a.do_hi && e_raw > a.emax);
+    const double a_e = clamped ? 0.0 : le;
+    *pl_out = 0.0;
+    if (!std::isfinite(pl) || !std::isfinite(pl0)) return 0.0; // tII = 0: eta = 2 eta_ini
+    double l_pl = a_e * (e_raw * e_raw) / (2.0 * pl * pl);
+    if (a.do_relax) {
+        l_pl += lPL;
+        const double l_pl0 = l_pl * a.relax * pl / pl0;
+        *pl_out = l_pl * (1.0 - a.relax) * pl / ePL;
+        l_pl = l_pl0;
+    }
+    return (tii > 0.0) ? l_pl * a.ex * pl0 / tii : 0.0;
+}
+SSPT_DH inline double adj_power_law(const AArgs &a, uint32_t o, double *pl_out)
+{
+    return adj_power_law_v(a, fat(a.li[A_E], o), fat(a.tII, o), fat(a.ini, o), a.do_relax ? fat(a.ePL, o) : 0.0,
+                           a.do_relax ? fat(a.li[A_PL], o) : 0.0, pl_out);
+}
+
+// objective weight test of the r-face (i, j, k): inside the inclusion
+SSPT_DH inline bool in_objective(const AArgs &a, int i, int j, int k)
+{
+    if (i < 1 || i > a.b.nr - 1) return false;
+    const double Rf = tload(a.t.tr, i * NTR + fused::TR_Rf);
+    const double x = Rf - a.obj_r, y = (a.t.th[j] - fused::PI_ / 2.0) * a.obj_r, z = a.t.ph[k] * a.obj_r;
+    return ((x * x + y * y) + z * z) < a.obj_rad;
+}
+
+struct ALds {
+    // phase A -> B
+    double wVR[ATJ][TK], wVT[ATJ][TK], wVP[ATJ][TK];
+    double QX[ATJ][TK], QY[ATJ][TK], QZ[ATJ][TK], GSM[ATJ][TK], ETA[ATJ][TK];
+    // phase B -> C
+    double dT[ATJ][TK], aT[ATJ][TK], dP[ATJ][TK];
+    double gRT[ATJ][TK], gRP[ATJ][TK], gTP[ATJ][TK], cRT[ATJ][TK], cRP[ATJ][TK], cTP[ATJ][TK];
+};
+
+struct AThread {
+    int tx, ty, j, k;
+    int jc, kc; // clamped into [0, nth] x [0, nph] for loads
+    bool own;
+    // carries from plane q-1 (or q, loaded ahead)
+    double DTq, wVRq, lVRq;          // DT(q), w_VR(q), lam_VR(q)
+    double wVTm, wVPm;               // w_Vth(q-1), w_Vph(q-1)
+    double QXm, QYm;                 // Q sX over (q-1) x (j-1, j); Q sY over (q-1) x (k-1, k)
+    double RTq, RTqj, RPq, RPqk;     // tau_Rth at (q, j), (q, j+1); tau_Rph at (q, k), (q, k+1)
+    double RTn, RTnj, RPn, RPnk;     // the same of box index q+1 (loaded this step)
+    double gRTm, eRTm, gRPm, eRPm;   // gs SM / eta sums of plane q-1 around the r-th / r-ph edges
+    double accVRn;                   // VR(q): term of centre q-1
+    double accVT, accVP, accE;       // Vth, Vph, eta of plane q-1 without the edges of box index q
+    // values of the current step
+    double DTq1, lVR1, wVR1, lVT, lVP, wVT, wVP, Q;
+    double GS, KB, e, tRR, tTT, tPP, tRT, tRP, tTP;
+    double lP, lRR, lTT, lPP, lRT, lRP, lTP;
+    double cw_dR, cw_dT, cw_dP, cw_aR, cw_aT, eC;
+    double gRT, gRP, gTP, cRT, cRP, cTP;
+    double nQXm, nQYm, ngRTm, neRTm, ngRPm, neRPm;
+};
+
+SSPT_DH inline void athread_init(AThread &T, const AArgs &a, int tx, int ty, int kb, int jb)
+{
+    T.tx = tx;
+    T.ty = SSPT_UNIFORM(ty);
+    T.k = kb - 1 + tx;
+    T.j = SSPT_UNIFORM(jb - 1 + ty);
+    T.jc = clampi(T.j, 0, a.b.nt);
+    T.kc = clampi(T.k, 0, a.b.np);
+    T.own = tx >= 1 && tx <= TK - 2 && ty >= 1 && ty <= ATJ - 2 && T.j <= a.b.nt && T.k <= a.b.np;
+    T.DTq = T.wVRq = T.lVRq = 0.0;
+    T.wVTm = T.wVPm = T.QXm = T.QYm = 0.0;
+    T.gRTm = T.eRTm = T.gRPm = T.eRPm = 0.0;
+    T.accVRn = T.accVT = T.accVP = T.accE = 0.0;
+}
+
+// DT(q), lam_VR(q) and w_VR(q) of the first plane of the march
+SSPT_DH inline void aprologue(AThread &T, const AArgs &a, int q)
+{
+    const BoxG &b = a.b;
+    const int qc = clampi(q, 0, b.nr);
+    T.DTq = fat(a.DT, b.o32(qc, T.jc, T.kc));
+    T.lVRq = fat(a.li[A_VR], b.o32(qc, T.jc, T.kc));
+    const double DTm = fat(a.DT, b.o32(clampi(q - 1, 0, b.nr), T.jc, T.kc));
+    T.wVRq = upd_VR(b, q, T.j, T.k) ? ((DTm + T.DTq) * 0.5) * T.lVRq : 0.0;
+    const int jc1 = clampi(T.jc + 1, 0, b.nt), kc1 = clampi(T.kc + 1, 0, b.np);
+    T.RTq = fat(a.tRT, b.o32(qc, T.jc, T.kc));
+    T.RTqj = fat(a.tRT, b.o32(qc, jc1, T.kc));
+    T.RPq = fat(a.tRP, b.o32(qc, T.jc, T.kc));
+    T.RPqk = fat(a.tRP, b.o32(qc, T.jc, kc1));
+}
+
+// Field values one step reads (plane q, r-face q+1 for VR), in two halves:
+// those of phase A (loaded during phase B of the step before) and those of
+// phase B (loaded at the start of phase A).  Each half's latency hides
+// behind a phase and a barrier, with one half in flight at a time.
+struct AStepA {
+    double DT1, lVR1, lVT, lVP, DTjm, DTkm;
+    double le, tii, ini, ePL, lPL, lb, sm, e;
+    double RT1, RT1j, RP1, RP1k;     // tau_Rth / tau_Rph of box index q+1 at (j, j+1) / (k, k+1)
+    double TP, TPk, TPj, TPjk;       // tau_thph of box index q at (j,k), (j,k+1), (j+1,k), (j+1,k+1)
+};
+struct AStepB {
+    double tRR, tTT, tPP;
+    double lP, lRR, lTT, lPP, lRT, lRP, lTP;
+};
+
+// The loads of step q (every position clamped into the box, so all are
+// unconditional).
+SSPT_DH inline void aloadA(AStepA &S, const AArgs &a, const AThread &T, int q)
+{
+    const BoxG &b = a.b;
+    const int qc = clampi(q, 0, b.nr);
+    const uint32_t o = b.o32(qc, T.jc, T.kc);
+    const uint32_t o1 = b.o32(clampi(q + 1, 0, b.nr), T.jc, T.kc);
+    S.DT1 = fat(a.DT, o1);
+    S.lVR1 = fat(a.li[A_VR], o1);
+    S.lVT = fat(a.li[A_VT], o);
+    S.lVP = fat(a.li[A_VP], o);
+    S.DTjm = fat(a.DT, b.o32(qc, clampi(T.jc - 1, 0, b.nt), T.kc));
+    S.DTkm = fat(a.DT, b.o32(qc, T.jc, clampi(T.kc - 1, 0, b.np)));
+    S.le = fat(a.li[A_E], o);
+    S.tii = fat(a.tII, o);
+    S.ini = fat(a.ini, o);
+    S.ePL = a.do_relax ? fat(a.ePL, o) : 0.0;
+    S.lPL = a.do_relax ? fat(a.li[A_PL], o) : 0.0;
+    const int jc1 = clampi(T.jc + 1, 0, b.nt), kc1 = clampi(T.kc + 1, 0, b.np);
+    S.RT1 = fat(a.tRT, o1);
+    S.RT1j = fat(a.tRT, b.o32(clampi(q + 1, 0, b.nr), jc1, T.kc));
+    S.RP1 = fat(a.tRP, o1);
+    S.RP1k = fat(a.tRP, b.o32(clampi(q + 1, 0, b.nr), T.jc, kc1));
+    S.TP = fat(a.tTP, o);
+    S.TPk = fat(a.tTP, b.o32(qc, T.jc, kc1));
+    S.TPj = fat(a.tTP, b.o32(qc, jc1, T.kc));
+    S.TPjk = fat(a.tTP, b.o32(qc, jc1, kc1));
+    // LB is read only next to the boundary (the condition depends on the
+    // position alone, so the load still issues with the others)
+    S.lb = next_to_boundary(b, q, T.j, T.k) ? fat(a.LB, o) : 0.0;
+    S.sm = fat(a.SM, o);
+    S.e = fat(a.eta, o);
+}
+SSPT_DH inline void aloadB(AStepB &S, const AArgs &a, const AThread &T, int q)
+{
+    const BoxG &b = a.b;
+    const uint32_t o = b.o32(clampi(q, 0, b.nr), T.jc, T.kc);
+    S.tRR = fat(a.tRR, o);
+    S.tTT = fat(a.tTT, o);
+    S.tPP = fat(a.tPP, o);
+    S.lP = fat(a.li[A_P], o);
+    S.lRR = fat(a.li[A_RR], o);
+    S.lTT = fat(a.li[A_TT], o);
+    S.lPP = fat(a.li[A_PP], o);
+    S.lRT = fat(a.li[A_RT], o);
+    S.lRP = fat(a.li[A_RP], o);
+    S.lTP = fat(a.li[A_TP], o);
+}
+
+// ---- phase A: w's of plane q (and w_VR of q+1), power-law adjoint and Q of
+// the cells of plane q, gs SM and eta of plane q, from the values S of step q
+SSPT_DH inline void aphaseA(AThread &T, ALds &L, const AArgs &a, const AStepA &S, int q, bool wr)
+{
+    const BoxG &b = a.b;
+    const int nr = b.nr;
+    const int j = T.j, k = T.k;
+    const int qc = clampi(q, 0, nr);
+    const uint32_t o = b.o32(qc, T.jc, T.kc);
+    T.DTq1 = S.DT1;
+    T.lVR1 = S.lVR1;
+    T.wVR1 = upd_VR(b, q + 1, j, k) ? ((T.DTq + T.DTq1) * 0.5) * T.lVR1 : 0.0;
+    T.lVT = S.lVT;
+    T.lVP = S.lVP;
+    const double le = S.le, tii = S.tii, ini = S.ini, ePL = S.ePL, lPL = S.lPL;
+    const double lb = S.lb, sm = S.sm;
+    // compute_TII!'s 4-point shear averages (times 1/4) of the cell
+    const double sX = 0.25 * ((((T.RTq + T.RTqj) + S.RT1) + S.RT1j) * 0.25);
+    const double sY = 0.25 * ((((T.RPq + T.RPqk) + S.RP1) + S.RP1k) * 0.25);
+    const double sZ = 0.25 * ((((S.TP + S.TPk) + S.TPj) + S.TPjk) * 0.25);
+    T.tRT = T.RTq;
+    T.tRP = T.RPq;
+    T.tTP = S.TP;
+    T.RTn = S.RT1;
+    T.RTnj = S.RT1j;
+    T.RPn = S.RP1;
+    T.RPnk = S.RP1k;
+    T.e = S.e;
+    T.wVT = upd_VT(b, q, j, k) ? ((S.DTjm + T.DTq) * 0.5) * T.lVT : 0.0;
+    T.wVP = upd_VP(b, q, j, k) ? ((S.DTkm + T.DTq) * 0.5) * T.lVP : 0.0;
+    // power-law adjoint and Q of the cell
+    double Q = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
+    if (is_cell(b, q, j, k)) {
+        double plo;
+        const double lt = adj_power_law_v(a, le, tii, ini, ePL, lPL, &plo);
+        if (a.do_relax && wr && T.own) fat(a.lo[A_PL], o) = plo;
+        if (inner_c(b, q, j, k)) {
+            // Q = lam_tII / tII (compute_TII!'s transpose; 0 where tII = 0)
+            const double s = lt + (next_to_boundary(b, q, j, k) ? lb : 0.0);
+            Q = (tii > 0.0) ? s * (1.0 / tii) : 0.0;
+            qx = Q * sX;
+            qy = Q * sY;
+            qz = Q * sZ;
+        }
+    }
+    T.Q = Q;
+    T.GS = a.gs * sm;
+    T.KB = a.kb * sm;
+    L.wVR[T.ty][T.tx] = T.wVRq;
+    L.wVT[T.ty][T.tx] = T.wVT;
+    L.wVP[T.ty][T.tx] = T.wVP;
+    L.QX[T.ty][T.tx] = qx;
+    L.QY[T.ty][T.tx] = qy;
+    L.QZ[T.ty][T.tx] = qz;
+    L.GSM[T.ty][T.tx] = T.GS;
+    L.ETA[T.ty][T.tx] = T.e;
+}
+
+// ---- phase B: adjoints of the new P and tau of centre q and of the edges
+// stored at box index q (the old-state adjoints: scaled by 1/den)
+SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S, int q, bool wr)
+{
+    T.tRR = S.tRR;
+    T.tTT = S.tTT;
+    T.tPP = S.tPP;
+    T.lP = S.lP;
+    T.lRR = S.lRR;
+    T.lTT = S.lTT;
+    T.lPP = S.lPP;
+    T.lRT = S.lRT;
+    T.lRP = S.lRP;
+    T.lTP = S.lTP;
+    const BoxG &b = a.b;
+    const TabsG<double> &t = a.t;
+    const int nr = b.nr, nt = b.nt;
+    const int j = T.j, k = T.k, tx = T.tx, ty = T.ty;
+    const int qt = clampi(q, 0, nr), qt1 = clampi(q + 1, 0, nr), qtm = clampi(q - 1, 0, nr);
+    const int jt = clampi(j, 0, nt), jt1 = clampi(j + 1, 0, nt), jtm = clampi(j - 1, 0, nt);
+    const uint32_t o = b.o32(qt, T.jc, T.kc);
+    const bool up = ty >= 1, lf = tx >= 1, dn = ty + 1 < ATJ, rt_ = tx + 1 < TK;
+    const double wR = T.wVRq, wT = T.wVT, wP = T.wVP;
+    const double wRj = up ? L.wVR[ty - 1][tx] : 0.0, wRk = lf ? L.wVR[ty][tx - 1] : 0.0;
+    const double wTj = dn ? L.wVT[ty + 1][tx] : 0.0, wTk = lf ? L.wVT[ty][tx - 1] : 0.0;
+    const double wPk = rt_ ? L.wVP[ty][tx + 1] : 0.0, wPj = up ? L.wVP[ty - 1][tx] : 0.0;
+    // ---- centre q
+    T.cw_dR = T.cw_dT = T.cw_dP = T.cw_aR = T.cw_aT = T.eC = 0.0;
+    if (is_cell(b, q, j, k)) {
+        const double wRi = T.wVR1;
+        const double iR = tload(t.tr, qt * NTR + fused::TR_invRf), iR1 = tload(t.tr, qt1 * NTR + fused::TR_invRf);
+        double sRR = 0.0, sTT = 0.0, sPP = 0.0;
+        sRR += wR * (a._dr + iR);
+        sTT -= 0.5 * wR * iR;
+        sPP -= 0.5 * wR * iR;
+        sRR += wRi * (-a._dr + iR1);
+        sTT -= 0.5 * wRi * iR1;
+        sPP -= 0.5 * wRi * iR1;
+        const double iRy = tload(t.tr, qt * NTR + fused::TR_invRc2);
+        const double ct = tload(t.tt, jt * NTT + fused::TT_cotf2), ct1 = tload(t.tt, jt1 * NTT + fused::TT_cotf2);
+        sTT += wT * (a._dth * iRy + 0.5 * ct * iRy);
+        sPP -= 0.5 * wT * ct * iRy;
+        sTT += wTj * (-a._dth * iRy + 0.5 * ct1 * iRy);
+        sPP -= 0.5 * wTj * ct1 * iRy;
+        sPP += (wP - wPk) * (tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_z) * a._dph);
+        double nRR = T.lRR + sRR, nTT = T.lTT + sTT, nPP = T.lPP + sPP;
+        if (inner_c(b, q, j, k)) {
+            const double qq = 0.5 * T.Q;
+            nRR += qq * T.tRR;
+            nTT += qq * T.tTT;
+            nPP += qq * T.tPP;
+        }
+        const double vP = T.lP - ((sRR + sTT) + sPP);
+        const double den = 1.0 + T.GS / T.e;
+        const double id = 1.0 / den;
+        const double vRR = nRR * id, vTT = nTT * id, vPP = nPP * id;
+        if (wr && T.own) {
+            fat(a.lo[A_P], o) = vP;
+            fat(a.lo[A_RR], o) = vRR;
+            fat(a.lo[A_TT], o) = vTT;
+            fat(a.lo[A_PP], o) = vPP;
+        }
+        // compute_P!'s centre stencil weights for vel
+        const double GS2 = 2.0 * T.GS;
+        const double xR = GS2 * vRR, xT = GS2 * vTT, xP = GS2 * vPP;
+        const double wdiv = -T.KB * vP - (xR + xT + xP) * (1.0 / 3.0);
+        const double iRc = tload(t.tr, qt * NTR + fused::TR_invRc);
+        T.cw_dR = xR + wdiv;
+        T.cw_dT = xT + wdiv;
+        T.cw_dP = xP + wdiv;
+        T.cw_aR = (xT + xP) * iRc + 2.0 * wdiv * iRc;
+        T.cw_aT = (xP + wdiv) * tload(t.tt, jt * NTT + fused::TT_cotc) * iRc;
+        T.eC = ((vRR * T.tRR + vTT * T.tTT) + vPP * T.tPP) * (T.GS / (T.e * T.e));
+    }
+    // ---- edges of box index q (only rows/lanes with a j-1 / k-1 neighbour)
+    T.gRT = T.gRP = T.gTP = T.cRT = T.cRP = T.cTP = 0.0;
+    const double r15m = 1.5 * tload(t.tr, qtm * NTR + fused::TR_invRc2), r15 = 1.5 * tload(t.tr, qt * NTR + fused::TR_invRc2);
+    if (up && in_RT(b, q, j, k)) {
+        const double iR = tload(t.tr, qt * NTR + fused::TR_invRf);
+        double s = wRj * (a._dth * iR + 0.5 * tload(t.tt, jtm * NTT + fused::TT_cotc2) * iR) +
+                   wR * (-a._dth * iR + 0.5 * tload(t.tt, jt * NTT + fused::TT_cotc2) * iR);
+        s += T.wVTm * (a._dr + r15m) + wT * (-a._dr + r15);
+        const double tq = (L.QX[ty][tx] + L.QX[ty - 1][tx]) + T.QXm;
+        const double g4 = ((T.gRTm + L.GSM[ty - 1][tx]) + L.GSM[ty][tx]) * 0.25;
+        const double e4 = ((T.eRTm + L.ETA[ty - 1][tx]) + L.ETA[ty][tx]) * 0.25;
+        const double v = ((T.lRT + s) + tq) * (1.0 / (1.0 + g4 / e4));
+        if (wr && T.own) fat(a.lo[A_RT], o) = v;
+        T.gRT = g4 * v;
+        T.cRT = (0.25 * T.tRT * (g4 / (e4 * e4))) * v;
+    }
+    if (lf && in_RP(b, q, j, k)) {
+        double s = (wRk - wR) * (tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_f) * a._dph);
+        s += T.wVPm * (a._dr + r15m) + wP * (-a._dr + r15);
+        const double tq = (L.QY[ty][tx] + L.QY[ty][tx - 1]) + T.QYm;
+        const double g4 = ((T.gRPm + L.GSM[ty][tx - 1]) + L.GSM[ty][tx]) * 0.25;
+        const double e4 = ((T.eRPm + L.ETA[ty][tx - 1]) + L.ETA[ty][tx]) * 0.25;
+        const double v = ((T.lRP + s) + tq) * (1.0 / (1.0 + g4 / e4));
+        if (wr && T.own) fat(a.lo[A_RP], o) = v;
+        T.gRP = g4 * v;
+        T.cRP = (0.25 * T.tRP * (g4 / (e4 * e4))) * v;
+    }
+    if (up && lf && in_TP(b, q, j, k)) {
+        double s = (wTk - wT) * (tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_y) * a._dph);
+        const double iR = tload(t.tr, qt * NTR + fused::TR_invRc2);
+        s += wPj * (a._dth * iR + tload(t.tt, jtm * NTT + fused::TT_cotc2) * iR) +
+             wP * (-a._dth * iR + tload(t.tt, jt * NTT + fused::TT_cotc2) * iR);
+        const double tq = ((L.QZ[ty][tx] + L.QZ[ty][tx - 1]) + L.QZ[ty - 1][tx]) + L.QZ[ty - 1][tx - 1];
+        const double g4 = (((L.GSM[ty - 1][tx - 1] + L.GSM[ty - 1][tx]) + L.GSM[ty][tx - 1]) + L.GSM[ty][tx]) * 0.25;
+        const double e4 = (((L.ETA[ty - 1][tx - 1] + L.ETA[ty - 1][tx]) + L.ETA[ty][tx - 1]) + L.ETA[ty][tx]) * 0.25;
+        const double v = ((T.lTP + s) + tq) * (1.0 / (1.0 + g4 / e4));
+        if (wr && T.own) fat(a.lo[A_TP], o) = v;
+        T.gTP = g4 * v;
+        T.cTP = (0.25 * T.tTP * (g4 / (e4 * e4))) * v;
+    }
+    // carries of plane q for the edges of box index q+1
+    T.nQXm = up ? L.QX[ty][tx] + L.QX[ty - 1][tx] : 0.0;
+    T.nQYm = lf ? L.QY[ty][tx] + L.QY[ty][tx - 1] : 0.0;
+    T.ngRTm = up ? L.GSM[ty - 1][tx] + L.GSM[ty][tx] : 0.0;
+    T.neRTm = up ? L.ETA[ty - 1][tx] + L.ETA[ty][tx] : 0.0;
+    T.ngRPm = lf ? L.GSM[ty][tx - 1] + L.GSM[ty][tx] : 0.0;
+    T.neRPm = lf ? L.ETA[ty][tx - 1] + L.ETA[ty][tx] : 0.0;
+    L.dT[ty][tx] = T.cw_dT;
+    L.aT[ty][tx] = T.cw_aT;
+    L.dP[ty][tx] = T.cw_dP;
+    L.gRT[ty][tx] = T.gRT;
+    L.gRP[ty][tx] = T.gRP;
+    L.gTP[ty][tx] = T.gTP;
+    L.cRT[ty][tx] = T.cRT;
+    L.cRP[ty][tx] = T.cRP;
+    L.cTP[ty][tx] = T.cTP;
+}
+
+// ---- phase C: lam_V' and lam_eta' contributions of plane q's sig values;
+// writes VR(q) and the finished Vth, Vph, eta of plane q-1
+SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr, bool wr_m)
+{
+    const BoxG &b = a.b;
+    const TabsG<double> &t = a.t;
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    const int j = T.j, k = T.k, tx = T.tx, ty = T.ty;
+    const int qt = clampi(q, 0, nr), jt = clampi(j, 0, nt), jt1 = clampi(j + 1, 0, nt);
+    const bool up = ty >= 1, lf = tx >= 1, dn = ty + 1 < ATJ, rt_ = tx + 1 < TK;
+    const double gRTj = dn ? L.gRT[ty + 1][tx] : 0.0, gRPk = rt_ ? L.gRP[ty][tx + 1] : 0.0;
+    const double gTPk = rt_ ? L.gTP[ty][tx + 1] : 0.0, gTPj = dn ? L.gTP[ty + 1][tx] : 0.0;
+    const double cRTj = dn ? L.cRT[ty + 1][tx] : 0.0, cRPk = rt_ ? L.cRP[ty][tx + 1] : 0.0;
+    const double iRe = tload(t.tr, qt * NTR + fused::TR_invRe);
+    const bool inj = j >= 0 && j < nt, ink = k >= 0 && k < np;
+    // ---- the finished Vth, Vph, eta of plane q-1 (edges of box index q)
+    if (wr_m && T.own && q >= 1) {
+        const uint32_t om = b.o32(q - 1, T.jc, T.kc);
+        const double mR = -a._dr - 0.5 * iRe;
+        if (j >= 0 && j <= nt && ink) fat(a.lo[A_VT], om) = T.accVT + T.gRT * mR;
+        if (inj && k >= 0 && k <= np) fat(a.lo[A_VP], om) = T.accVP + T.gRP * mR;
+        if (inj && ink) fat(a.lo[A_E], om) = ((T.accE + T.cRT) + cRTj) + (T.cRP + cRPk);
+    }
+    // ---- VR(q): complete
+    if (q <= nr && inj && ink) {
+        double v = T.lVRq + T.accVRn;
+        if (q <= nr - 1) v += -T.cw_dR * a._dr + 0.5 * T.cw_aR;
+        const double dthR = a._dth * iRe;
+        v += T.gRT * dthR;
+        v -= gRTj * dthR;
+        const double re = tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_e) * a._dph;
+        v += T.gRP * re;
+        v -= gRPk * re;
+        if (a.bJ && in_objective(a, q, j, k)) v += fat(a.bJ, b.o32(qt, T.jc, T.kc));
+        if (wr && T.own) fat(a.lo[A_VR], b.o32(qt, T.jc, T.kc)) = v;
+    }
+    T.accVRn = is_cell(b, q, j, k) ? T.cw_dR * a._dr + 0.5 * T.cw_aR : 0.0;
+    // ---- Vth(q), Vph(q), eta(q) without the edges of box index q+1
+    double vT = 0.0, vP = 0.0, vE = 0.0;
+    if (q < nr) {
+        const double pR = a._dr - 0.5 * iRe;
+        if (j >= 0 && j <= nt && ink) {
+            double v = T.lVT;
+            const double dth_R = a._dth * tload(t.tr, qt * NTR + fused::TR_invRc);
+            if (up) v += L.dT[ty - 1][tx] * dth_R + 0.5 * L.aT[ty - 1][tx];
+            v += -T.cw_dT * dth_R + 0.5 * T.cw_aT;
+            v += T.gRT * pR;
+            const double r4 = tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_c4) * a._dph;
+            v += T.gTP * r4;
+            v -= gTPk * r4;
+            vT = v;
+        }
+        if (inj && k >= 0 && k <= np) {
+            double v = T.lVP;
+            const double rc = tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_c) * a._dph;
+            if (lf) v += L.dP[ty][tx - 1] * rc;
+            v -= T.cw_dP * rc;
+            v += T.gRP * pR;
+            const double iR4 = tload(t.tr, qt * NTR + fused::TR_invRc4);
+            v += T.gTP * (a._dth * iR4 - 0.5 * tload(t.tt, jt * NTT + fused::TT_cotf4) * iR4);
+            v += gTPj * (-a._dth * iR4 - 0.5 * tload(t.tt, jt1 * NTT + fused::TT_cotf4) * iR4);
+            vP = v;
+        }
+        if (inj && ink) {
+            const double cTPk = rt_ ? L.cTP[ty][tx + 1] : 0.0, cTPj = dn ? L.cTP[ty + 1][tx] : 0.0;
+            const double cTPjk = (dn && rt_) ? L.cTP[ty + 1][tx + 1] : 0.0;
+            vE = ((((T.eC + T.cRT) + T.cRP) + T.cTP) + (cRTj + cRPk)) + ((cTPk + cTPj) + cTPjk);
+        }
+    }
+    T.accVT = vT;
+    T.accVP = vP;
+    T.accE = vE;
+    // ---- carries for step q+1
+    T.DTq = T.DTq1;
+    T.RTq = T.RTn;
+    T.RTqj = T.RTnj;
+    T.RPq = T.RPn;
+    T.RPqk = T.RPnk;
+    T.wVRq = T.wVR1;
+    T.lVRq = T.lVR1;
+    T.wVTm = T.wVT;
+    T.wVPm = T.wVP;
+    T.QXm = T.nQXm;
+    T.QYm = T.nQYm;
+    T.gRTm = T.ngRTm;
+    T.eRTm = T.neRTm;
+    T.gRPm = T.ngRPm;
+    T.eRPm = T.neRPm;
+    (void)up;
+}
+
+// Sum of lam_tII over the boundary cells copy_BC! fills from interior cell
+// (i, j, k) (the cells at its clamped preimage other than itself).
+SSPT_DH inline double lb_sum(const AArgs &a, int i, int j, int k)
+{
+    const BoxG &b = a.b;
+    int di[3] = {0, 0, 0}, dj[3] = {0, 0, 0}, dk[3] = {0, 0, 0};
+    int ni = 1, nj = 1, nk = 1;
+    if (i == 1) di[ni++] = -1;
+    if (i == b.nr - 2) di[ni++] = 1;
+    if (j == 1) dj[nj++] = -1;
+    if (j == b.nt - 2) dj[nj++] = 1;
+    if (k == 1) dk[nk++] = -1;
+    if (k == b.np - 2) dk[nk++] = 1;
+    double s = 0.0;
+    for (int p = 0; p < ni; ++p)
+        for (int r = 0; r < nj; ++r)
+            for (int w = 0; w < nk; ++w) {
+                if (p == 0 && r == 0 && w == 0) continue;
+                double plo;
+                s += adj_power_law(a, b.o32(i + di[p], j + dj[r], k + dk[w]), &plo);
+            }
+    return s;
+}
+
+} // namespace adjf
+} // namespace sspt

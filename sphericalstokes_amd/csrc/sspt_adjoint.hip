@@ -27,8 +27,10 @@
 //      transposes) and lam_eta (its viscosity dependence).
 #include "sspt_internal.h"
 #include "sspt_launch.h"
+#include "sspt_adj_fused_body.h"
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -43,7 +45,8 @@ enum {
     AD_SID = 32, AD_SGS = 33, AD_SKB = 34, AD_SAE = 35, AD_SX = 36, AD_SY = 37, AD_SZ = 38, AD_SIT = 39,
     AD_GRT = 40, AD_IDRT = 41, AD_CRT = 42, AD_GRP = 43, AD_IDRP = 44, AD_CRP = 45,
     AD_GTP = 46, AD_IDTP = 47, AD_CTP = 48,
-    AD_NBOX = 49
+    AD_LB = 49,    // boundary power-law sums of the cells next to the boundary (r-marching kernel)
+    AD_NBOX = 50
 };
 static_assert(AD_NBOX == SS_ADJ_NBOX, "adjoint box count");
 enum { L_VR, L_VT, L_VP, L_P, L_RR, L_TT, L_PP, L_RT, L_RP, L_TP, L_E, L_PL, L_N };
@@ -636,6 +639,83 @@ __global__ void k_adj_zero(double *p, size_t n)
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = 0.0;
 }
 
+// ------------------------------------------------------------------ r-marching form
+// The whole of steps 2-5 in one launch (sspt_adj_fused_body.h): a block of
+// 64 x ATJ columns marches along r through its chunk of planes.  Tiles are
+// dealt so that each XCD gets a contiguous run of them.
+// The kernel's arguments re-read from the kernarg segment.  The pointer is
+// passed through an empty asm statement at each use, so the compiler cannot
+// hoist the loads of the argument fields out of the plane loop: each phase
+// loads (with scalar loads) the 20-odd field bases it uses, instead of all
+// 40-odd staying live in scalar registers across the loop and spilling.
+__device__ __forceinline__ const adjf::AArgs &kargs()
+{
+    uint64_t v = (uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(v));
+    return *(const adjf::AArgs *)(const __attribute__((address_space(4))) adjf::AArgs *)v;
+}
+
+__global__ void __launch_bounds__(fused::TK * adjf::ATJ) k_adj_fused(adjf::AArgs a, int nx, int ny, int nz)
+{
+    using namespace adjf;
+    __shared__ ALds L;
+    const int T_ = nx * ny * nz;
+    const int per = (T_ + 7) / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    const int bx = t % nx, by = (t / nx) % ny, bz = t / (nx * ny);
+    if (bz >= nz) return; // padding slots of the remap (uniform per block)
+    const int kb = bx * AOK, jb = by * AOJ;
+    const int s0 = bz * a.rchunk, s1 = min(s0 + a.rchunk, a.b.nr + 1);
+    const int start = max(s0 - 1, 0), stop = min(s1, a.b.nr);
+    AThread T;
+    athread_init(T, a, threadIdx.x, threadIdx.y, kb, jb);
+    aprologue(T, a, start);
+    AStepA na;
+    aloadA(na, a, T, start);
+    for (int q = start; q <= stop; ++q) {
+        const bool wr = q >= s0 && q < s1, wr_m = q - 1 >= s0 && q - 1 < s1;
+        const AStepA ca = na;
+        AStepB cb;
+        aloadB(cb, kargs(), T, q);
+        aphaseA(T, L, kargs(), ca, q, wr);
+        __syncthreads();
+        aloadA(na, kargs(), T, q + 1); // next step's (clamped into the box at the end)
+        aphaseB(T, L, kargs(), cb, q, wr);
+        __syncthreads();
+        // phase C reads only the phase-B arrays; the next phase A writes only
+        // the phase-A arrays, so no barrier is needed between them
+        aphaseC(T, L, kargs(), q, wr, wr_m);
+    }
+}
+
+// LB of the interior cells next to the boundary: the power-law adjoints of
+// the boundary cells copy_BC! fills from each.  Families: 0 -> i in {1,
+// nr-2}; 1 -> j in {1, nt-2} (i inside); 2 -> k in {1, np-2} (i, j inside);
+// each such cell once.
+__global__ void __launch_bounds__(256) k_adj_lb(adjf::AArgs a, double *LB)
+{
+    const adjf::BoxG &b = a.b;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y * blockDim.y + threadIdx.y;
+    const int fam = blockIdx.z, nr = b.nr, nt = b.nt, np = b.np;
+    for (int side = 0; side < 2; ++side) {
+        int i, j, k;
+        if (fam == 0) {
+            i = side ? nr - 2 : 1; j = v; k = u;
+            if (side && nr - 2 == 1) continue;
+            if (j < 1 || j > nt - 2 || k < 1 || k > np - 2) continue;
+        } else if (fam == 1) {
+            j = side ? nt - 2 : 1; i = v; k = u;
+            if (side && nt - 2 == 1) continue;
+            if (i < 2 || i > nr - 3 || k < 1 || k > np - 2) continue;
+        } else {
+            k = side ? np - 2 : 1; i = v; j = u;
+            if (side && np - 2 == 1) continue;
+            if (i < 2 || i > nr - 3 || j < 2 || j > nt - 3) continue;
+        }
+        LB[b.off(i, j, k)] = adjf::lb_sum(a, i, j, k);
+    }
+}
+
 } // namespace sspt
 
 // ======================================================================== API
@@ -715,6 +795,42 @@ AdjArgs make_adj_args(ss_ctx *c, const ss_params &p)
     return a;
 }
 
+// Arguments of the r-marching kernel for the sets of a (the same step)
+adjf::AArgs fused_args(ss_ctx *c, const AdjArgs &a, bool with_b)
+{
+    adjf::AArgs f{};
+    const Box &b = c->box;
+    f.b = adjf::BoxG{b.nr, b.nt, b.np, b.sj, b.si};
+    f.t = fused::TabsG<double>{c->ftr, c->ftt, c->ft2, c->tab.th, c->tab.ph};
+    for (int q = 0; q < L_N; ++q) {
+        f.li[q] = a.in.f[q];
+        f.lo[q] = a.out.f[q];
+    }
+    f.tRR = a.tRR; f.tTT = a.tTT; f.tPP = a.tPP; f.tRT = a.tRT; f.tRP = a.tRP; f.tTP = a.tTP;
+    f.eta = a.eta; f.ini = a.ini; f.ePL = a.ePL; f.tII = a.tII; f.DT = a.DT; f.SM = a.SM;
+    f.LB = c->adj[AD_LB];
+    f.bJ = with_b ? c->adj[AD_BJ] : nullptr;
+    f.obj_r = c->adj_obj_r;
+    f.obj_rad = c->adj_obj_rad;
+    f._dr = a._dr; f._dth = a._dth; f._dph = a._dph; f.kb = a.kb; f.gs = a.gs; f.tC = a.tC; f.ex = a.ex;
+    f.relax = a.relax; f.emin = a.emin; f.emax = a.emax;
+    f.do_relax = a.do_relax; f.do_lo = a.do_lo; f.do_hi = a.do_hi;
+    return f;
+}
+
+// r-chunk of the marching kernel: about 6 000 workgroups (a few per CU at
+// a time, several rounds) with chunks of at least 4 planes
+int adj_rchunk(int nplanes, int tiles)
+{
+    if (const char *e = std::getenv("SSPT_ADJ_RCHUNK")) {
+        const int v = std::atoi(e);
+        if (v >= 1) return std::min(v, nplanes);
+    }
+    int nz = (6144 + tiles - 1) / tiles;
+    nz = std::max(1, std::min(nz, nplanes / 4));
+    return (nplanes + nz - 1) / nz;
+}
+
 // lam_out = T^T lam_in (+ bJ on VR if given); gradients when a.grad
 void launch_adj_step(ss_ctx *c, AdjArgs a, bool with_b)
 {
@@ -724,6 +840,21 @@ void launch_adj_step(ss_ctx *c, AdjArgs a, bool with_b)
     hipLaunchKernelGGL(k_adj_bc_ph, G2, B2, 0, c->stream, a);
     hipLaunchKernelGGL(k_adj_bc_th, G2, B2, 0, c->stream, a);
     hipLaunchKernelGGL(k_adj_bc_R, G2, B2, 0, c->stream, a);
+    if (c->adj_fused) {
+        adjf::AArgs f = fused_args(c, a, with_b);
+        hipLaunchKernelGGL(k_adj_lb, dim3((m + 63) / 64, (m + 3) / 4, 3), B2, 0, c->stream, f, c->adj[AD_LB]);
+        const int nx = (b.np + 1 + adjf::AOK - 1) / adjf::AOK, ny = (b.nt + 1 + adjf::AOJ - 1) / adjf::AOJ;
+        f.rchunk = adj_rchunk(b.nr + 1, nx * ny);
+        const int nz = (b.nr + 1 + f.rchunk - 1) / f.rchunk;
+        const int T = nx * ny * nz;
+        hipLaunchKernelGGL(k_adj_fused, dim3(((T + 7) / 8) * 8), dim3(fused::TK, adjf::ATJ), 0, c->stream, f, nx, ny,
+                           nz);
+        if (a.grad) {
+            hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+            hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+        }
+        return;
+    }
     hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
     if (a.grad) hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
     hipLaunchKernelGGL(k_adj_tii, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
@@ -756,6 +887,7 @@ int ss_adjoint_prepare(ss_ctx *c, const ss_params *p)
     if (hipSetDevice(c->device) != hipSuccess) return afail(c, SS_ERR_STATE, "hipSetDevice");
     int rc = ensure_adj(c);
     if (rc != SS_OK) return rc;
+    if (const char *e = std::getenv("SSPT_ADJ_FUSED")) c->adj_fused = std::atoi(e) != 0;
     AdjArgs a = make_adj_args(c, *p);
     const Box &b = c->box;
     hipLaunchKernelGGL(k_adj_prep_centre, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
@@ -787,6 +919,8 @@ int ss_adjoint_objective(ss_ctx *c, double r, double radius, double *J)
     dummy.eta_max = INFINITY;
     AdjArgs a = make_adj_args(c, dummy);
     a.bJ = c->adj[AD_BJ];
+    c->adj_obj_r = r;
+    c->adj_obj_rad = radius;
     hipLaunchKernelGGL(k_adj_obj_partial, dim3(OBJ_NBLK), dim3(OBJ_BLOCK), 0, c->stream, a, r, radius, c->adj_part);
     hipLaunchKernelGGL(k_adj_obj_final, dim3(1), dim3(OBJ_BLOCK), 0, c->stream, c->adj_part, OBJ_NBLK,
                        c->adj_part + 2 * OBJ_NBLK);

@@ -49,6 +49,18 @@ __device__ __forceinline__ void tile_of(int nx, int ny, int nz, int &bx, int &by
     bz = t / (nx * ny);
 }
 
+// The kernel's arguments re-read from the kernarg segment.  The pointer is
+// passed through an empty asm statement at each use, so the compiler cannot
+// hoist the loads of the argument fields out of the r-march: each phase
+// loads (with scalar loads) the field bases it uses, instead of all of them
+// staying live in scalar registers across the loop and spilling.
+template <typename R> __device__ __forceinline__ const Args<R> &kargs()
+{
+    uint64_t v = (uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(v));
+    return *(const Args<R> *)(const __attribute__((address_space(4))) Args<R> *)v;
+}
+
 template <typename R, int FL>
 __global__ void __launch_bounds__(TK * TJB<R>) k_fused_main(Args<R> a, int nx, int ny, int nz)
 {
@@ -64,15 +76,15 @@ __global__ void __launch_bounds__(TK * TJB<R>) k_fused_main(Args<R> a, int nx, i
     thread_prologue<R, FL>(T, a, start);
     for (int s = start; s < s1; ++s) {
         const bool wr = s >= s0;
-        phase1<R, FL>(T, L, a, s);
+        phase1<R, FL>(T, L, kargs<R>(), s);
         __syncthreads();
-        phase2<R, FL>(T, L, a, s, wr);
+        phase2<R, FL>(T, L, kargs<R>(), s, wr);
         __syncthreads();
-        phase3<R, FL>(T, L, a, s, wr);
+        phase3<R, FL>(T, L, kargs<R>(), s, wr);
         __syncthreads();
         // phase4 reads only STT/SPP/DT/TP/RT/RP; the next phase1 writes only
         // VR/VT/VP/ETA, so no barrier is needed between them.
-        phase4<R, FL>(T, L, a, s, wr);
+        phase4<R, FL>(T, L, kargs<R>(), s, wr);
     }
     if (FL & F_CHECK) flush_max(a, T.mdv, T.mv, threadIdx.x);
 }

@@ -73,7 +73,7 @@ struct FieldSet {
 enum { SS_F32_ETA = 20, SS_F32_GP = 21, SS_F32_FVR = 22, SS_F32_FVT = 23, SS_F32_FVP = 24, SS_F32_NBOX = 25 };
 
 // FP64 boxes of the adjoint solve (sspt_adjoint.hip lists them)
-enum { SS_ADJ_NBOX = 49 };
+enum { SS_ADJ_NBOX = 50 };
 
 } // namespace sspt
 
@@ -101,6 +101,8 @@ struct ss_ctx {
     double *adj_part = nullptr;     // partial sums of the objective
     bool adj_ready = false;
     int adj_cur = 0;                // lam set holding the current iterate
+    double adj_obj_r = 0.0, adj_obj_rad = 0.0; // inclusion of the objective (ss_adjoint_objective)
+    bool adj_fused = true;          // r-marching adjoint kernel (SSPT_ADJ_FUSED=0: the per-family kernels)
     bool grid_set = false;
     double *staging = nullptr;      // upload/download staging (largest field)
     size_t staging_elems = 0;

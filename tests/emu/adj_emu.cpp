@@ -1,0 +1,176 @@
+// adj_emu.cpp -- host emulation of the r-marching adjoint kernel
+// (sphericalstokes_amd/csrc/sspt_adj_fused_body.h), test support only.
+//
+// Runs the same phase functions as the device kernel, looping over the
+// threads of a block between the barriers, on box-layout host arrays.  The
+// BC fold and the boundary power-law sums that the
+// library computes in separate small kernels are restated here as loops.
+#define SSPT_DH
+#define SSPT_UNIFORM(x) (x)
+#define SSPT_CONSTANT_PTR(T, p) (p)
+#include "../../sphericalstokes_amd/csrc/sspt_adj_fused_body.h"
+#include "../../sphericalstokes_amd/csrc/sspt_tables.h"
+#include "../../include/sspt.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+using namespace sspt::adjf;
+
+namespace {
+
+struct HostBox {
+    int nr, nt, np;
+    long long sj, si;
+};
+
+// transposes of BC_ph!, BC_th!, BC_R! (in this order) on lam_V in place
+void bc_fold(const BoxG &b, int bc_mode, double *VR, double *VT, double *VP)
+{
+    const int nr = b.nr, nt = b.nt, np = b.np;
+    const bool cp = bc_mode != SS_BC_SHEAR;
+    const long long sj = b.sj, si = b.si;
+    for (int i = 0; i <= nr; ++i)
+        for (int j = 0; j <= nt; ++j) {
+            if (i < nr && j < nt) {
+                const long long lo = b.off(i, j, 0), hi = b.off(i, j, np);
+                VP[lo + 1] -= VP[lo]; VP[lo] = 0.0;
+                VP[hi - 1] -= VP[hi]; VP[hi] = 0.0;
+            }
+            if (i < nr && j <= nt) {
+                const long long lo = b.off(i, j, 0), hi = b.off(i, j, np - 1);
+                if (cp) { VT[lo + 1] += VT[lo]; VT[hi - 1] += VT[hi]; }
+                VT[lo] = 0.0; VT[hi] = 0.0;
+            }
+            if (i <= nr && j < nt) {
+                const long long lo = b.off(i, j, 0), hi = b.off(i, j, np - 1);
+                if (cp) { VR[lo + 1] += VR[lo]; VR[hi - 1] += VR[hi]; }
+                VR[lo] = 0.0; VR[hi] = 0.0;
+            }
+        }
+    for (int i = 0; i <= nr; ++i)
+        for (int k = 0; k <= np; ++k) {
+            if (i < nr && k <= np) {
+                const long long lo = b.off(i, 0, k), hi = b.off(i, nt - 1, k);
+                if (cp) { VP[lo + sj] += VP[lo]; VP[hi - sj] += VP[hi]; }
+                VP[lo] = 0.0; VP[hi] = 0.0;
+            }
+            if (i < nr && k < np) {
+                const long long lo = b.off(i, 0, k), hi = b.off(i, nt, k);
+                VT[lo + sj] -= VT[lo]; VT[lo] = 0.0;
+                VT[hi - sj] -= VT[hi]; VT[hi] = 0.0;
+            }
+            if (i <= nr && k < np) {
+                const long long lo = b.off(i, 0, k), hi = b.off(i, nt - 1, k);
+                if (cp) { VR[lo + sj] += VR[lo]; VR[hi - sj] += VR[hi]; }
+                VR[lo] = 0.0; VR[hi] = 0.0;
+            }
+        }
+    for (int j = 0; j <= nt; ++j)
+        for (int k = 0; k <= np; ++k) {
+            if (j < nt && k <= np) {
+                const long long lo = b.off(0, j, k), hi = b.off(nr - 1, j, k);
+                VP[lo + si] += VP[lo]; VP[lo] = 0.0;
+                VP[hi - si] += VP[hi]; VP[hi] = 0.0;
+            }
+            if (j <= nt && k < np) {
+                const long long lo = b.off(0, j, k), hi = b.off(nr - 1, j, k);
+                VT[lo + si] += VT[lo]; VT[lo] = 0.0;
+                VT[hi - si] += VT[hi]; VT[hi] = 0.0;
+            }
+            if (j < nt && k < np) {
+                const long long lo = b.off(0, j, k), hi = b.off(nr, j, k);
+                VR[lo + si] -= VR[lo]; VR[lo] = 0.0;
+                VR[hi - si] -= VR[hi]; VR[hi] = 0.0;
+            }
+        }
+}
+
+} // namespace
+
+extern "C" {
+
+// One adjoint step lam_out = T^T lam_in on box-layout buffers (shape
+// (nr+1) x (nth+1) x pitch, phi fastest).  state: tau_RR tau_thth tau_phph
+// tau_Rth tau_Rph tau_thph eta eta_ini eta_PL tII DT_R etaSM; lam_in is
+// BC-folded in place; lam_out must be zero-initialised.
+int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const double *th1, const double *ph1,
+                 const ss_params *p, double **state, double **lam_in, double **lam_out, int rchunk)
+{
+    HostBox hb{nr, nt, np, pitch, (long long)(nt + 1) * pitch};
+    const BoxG b{nr, nt, np, hb.sj, hb.si};
+    std::vector<double> tab, ftab;
+    sspt::build_tables(hb, R1, th1, ph1, tab);
+    sspt::build_fused_tables(hb, tab, ftab);
+    const size_t n1 = nr + 1, n2 = nt + 1;
+    AArgs a{};
+    a.b = b;
+    a.t.tr = ftab.data();
+    a.t.tt = a.t.tr + sspt::FT_NTR * n1;
+    a.t.t2 = a.t.tt + sspt::FT_NTT * n2;
+    a.t.th = tab.data() + 10 * n1 + 4 * n2 + 6 * n1 * n2;
+    a.t.ph = a.t.th + n2;
+    static constexpr double PI = 3.14159265358979323846;
+    a._dr = p->_dr; a._dth = p->_dth; a._dph = p->_dph;
+    a.kb = (((p->dmp * 2.0) * PI) * p->vpdt) / p->lr;
+    a.gs = ((4.0 * PI) * p->vpdt) / p->lr;
+    a.tC = p->tau_C; a.ex = 1.0 - p->n_exp; a.relax = p->relax; a.emin = p->eta_min; a.emax = p->eta_max;
+    a.do_relax = p->relax != 1.0;
+    a.do_lo = p->eta_min > 0.0;
+    a.do_hi = p->eta_max < INFINITY;
+    a.tRR = state[0]; a.tTT = state[1]; a.tPP = state[2]; a.tRT = state[3]; a.tRP = state[4]; a.tTP = state[5];
+    a.eta = state[6]; a.ini = state[7]; a.ePL = state[8]; a.tII = state[9]; a.DT = state[10]; a.SM = state[11];
+    for (int q = 0; q < A_N; ++q) {
+        a.li[q] = lam_in[q];
+        a.lo[q] = lam_out[q];
+    }
+    bc_fold(b, p->bc_mode, lam_in[A_VR], lam_in[A_VT], lam_in[A_VP]);
+    const size_t nbox = (size_t)(nr + 1) * (nt + 1) * pitch;
+    std::vector<double> LB(nbox, std::nan(""));
+    for (int i = 1; i <= nr - 2; ++i)
+        for (int j = 1; j <= nt - 2; ++j)
+            for (int k = 1; k <= np - 2; ++k)
+                if (next_to_boundary(b, i, j, k)) LB[b.off(i, j, k)] = lb_sum(a, i, j, k);
+    a.LB = LB.data();
+    a.bJ = nullptr;
+    const int rc = rchunk > 0 ? rchunk : nr + 1;
+    a.rchunk = rc;
+    const int nx = (np + 1 + AOK - 1) / AOK, ny = (nt + 1 + AOJ - 1) / AOJ, nz = (nr + 1 + rc - 1) / rc;
+    std::unique_ptr<ALds> L(new ALds());
+    std::vector<AThread> th(TK * ATJ);
+    std::vector<AStepA> na(TK * ATJ), ca(TK * ATJ);
+    std::vector<AStepB> cb(TK * ATJ);
+    for (int bz = 0; bz < nz; ++bz)
+        for (int by = 0; by < ny; ++by)
+            for (int bx = 0; bx < nx; ++bx) {
+                memset(L.get(), 0xff, sizeof(ALds)); // NaN garbage: unset LDS reads must not reach outputs
+                const int kb = bx * AOK, jb = by * AOJ;
+                const int s0 = bz * rc, s1 = std::min(s0 + rc, nr + 1);
+                const int start = std::max(s0 - 1, 0), stop = std::min(s1, nr);
+                for (int ty = 0; ty < ATJ; ++ty)
+                    for (int tx = 0; tx < TK; ++tx) {
+                        AThread &T = th[ty * TK + tx];
+                        athread_init(T, a, tx, ty, kb, jb);
+                        aprologue(T, a, start);
+                        aloadA(na[ty * TK + tx], a, T, start);
+                    }
+                for (int q = start; q <= stop; ++q) {
+                    const bool wr = q >= s0 && q < s1, wr_m = q - 1 >= s0 && q - 1 < s1;
+                    for (size_t t = 0; t < th.size(); ++t) {
+                        ca[t] = na[t];
+                        aloadB(cb[t], a, th[t], q);
+                        aphaseA(th[t], *L, a, ca[t], q, wr);
+                    }
+                    for (size_t t = 0; t < th.size(); ++t) {
+                        aloadA(na[t], a, th[t], q + 1);
+                        aphaseB(th[t], *L, a, cb[t], q, wr);
+                    }
+                    for (auto &T : th) aphaseC(T, *L, a, q, wr, wr_m);
+                }
+            }
+    return 0;
+}
+
+} // extern "C"

@@ -89,3 +89,45 @@ def run(setup, fields, params, niter, check_last=False, rchunk=0, paired=False):
     shapes = {n: fields[n].shape for n in FIELDS}
     reds = np.array(list(red), dtype=np.uint64).view(np.float64)
     return from_box(setup, box, shapes), reds
+
+
+ADJ = ["VR", "Vth", "Vph", "P", "tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph",
+       "eta", "eta_PL"]
+ADJ_STATE = ["tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph", "eta", "eta_ini", "eta_PL", "tII",
+             "DT_R", "etaSM"]
+
+
+def _box1(setup, name, a):
+    nr, nt, np_ = setup.nr, setup.nth, setup.nph
+    b = np.zeros((nr + 1, nt + 1, pitch_of(np_)))
+    o = 1 if name in SHEAR else 0
+    b[o:o + a.shape[0], o:o + a.shape[1], o:o + a.shape[2]] = a
+    return b
+
+
+def adj_step(setup, state, lam, params, rchunk=0):
+    """One application of the r-marching adjoint kernel on the host: state
+    (forward fields by name), lam (adjoint fields by name) -> new lam."""
+    L = lib()
+    DP = C.POINTER(C.c_double)
+    if not hasattr(L, "_adj_sig"):
+        L.emu_adj_step.restype = C.c_int
+        L.emu_adj_step.argtypes = [C.c_int] * 4 + [DP] * 3 + [C.POINTER(ss_params), C.POINTER(DP), C.POINTER(DP),
+                                                              C.POINTER(DP), C.c_int]
+        L._adj_sig = True
+    sb = [_box1(setup, n, state[n]) for n in ADJ_STATE]
+    li = [_box1(setup, n, lam[n]) for n in ADJ]
+    lo = [np.zeros_like(x) for x in li]
+    ps = (DP * len(sb))(*[x.ctypes.data_as(DP) for x in sb])
+    pi = (DP * len(li))(*[x.ctypes.data_as(DP) for x in li])
+    po = (DP * len(lo))(*[x.ctypes.data_as(DP) for x in lo])
+    sp = ss_params.from_params(params)
+    R, th, ph = (np.ascontiguousarray(x, dtype=np.float64) for x in (setup.R, setup.th, setup.ph))
+    L.emu_adj_step(setup.nr, setup.nth, setup.nph, pitch_of(setup.nph), R.ctypes.data_as(DP), th.ctypes.data_as(DP),
+                   ph.ctypes.data_as(DP), C.byref(sp), ps, pi, po, int(rchunk))
+    out = {}
+    for n, x in zip(ADJ, lo):
+        o = 1 if n in SHEAR else 0
+        s = lam[n].shape
+        out[n] = np.asfortranarray(x[o:o + s[0], o:o + s[1], o:o + s[2]])
+    return out

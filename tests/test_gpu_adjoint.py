@@ -49,10 +49,13 @@ STEP_CASES = {
 }
 
 
+@pytest.mark.parametrize("path", ["marching", "per_family"])
 @pytest.mark.parametrize("case", list(STEP_CASES))
-def test_adjoint_step_matches_reference(Solver, case):
+def test_adjoint_step_matches_reference(Solver, case, path, monkeypatch):
     """One application of T^T (and of (dPhi/dm)^T) on random adjoint fields at
-    a random state (T^T is defined at any state with its step sizes held)."""
+    a random state (T^T is defined at any state with its step sizes held),
+    by the r-marching kernel and by the per-family kernels."""
+    monkeypatch.setenv("SSPT_ADJ_FUSED", "1" if path == "marching" else "0")
     st = setups.diapir_sector(11, nth=13, nph=70, **STEP_CASES[case])
     setups.random_state(st, seed=11)
     o = Oracle(st)
@@ -62,7 +65,7 @@ def test_adjoint_step_matches_reference(Solver, case):
     if st.params.relax == 1.0:
         lam["eta_PL"][:] = 0.0
     ref, g_rho, g_ini = o.adj_step(lam, grads=True)
-    say("adjoint step", case)
+    say("adjoint step", case, path)
     with _load(Solver, st, o.f) as s:
         s.adjoint_prepare(st.params)
         s.adjoint_zero()
