@@ -887,6 +887,11 @@ int ss_adjoint_prepare(ss_ctx *c, const ss_params *p)
     if (hipSetDevice(c->device) != hipSuccess) return afail(c, SS_ERR_STATE, "hipSetDevice");
     int rc = ensure_adj(c);
     if (rc != SS_OK) return rc;
+    // the r-marching kernel from about 255^3 cells up; below, the per-family
+    // kernels (one thread per point) keep more of the GPU busy (measured:
+    // 0.50 vs 0.37 ms at 127^3, 2.40 vs 2.48 ms at 255^3, 14.0 vs 19.6 ms at
+    // 511^3)
+    c->adj_fused = (double)c->box.nr * c->box.nt * c->box.np >= 1.6e7;
     if (const char *e = std::getenv("SSPT_ADJ_FUSED")) c->adj_fused = std::atoi(e) != 0;
     AdjArgs a = make_adj_args(c, *p);
     const Box &b = c->box;

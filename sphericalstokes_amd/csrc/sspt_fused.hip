@@ -49,18 +49,6 @@ __device__ __forceinline__ void tile_of(int nx, int ny, int nz, int &bx, int &by
     bz = t / (nx * ny);
 }
 
-// The kernel's arguments re-read from the kernarg segment.  The pointer is
-// passed through an empty asm statement at each use, so the compiler cannot
-// hoist the loads of the argument fields out of the r-march: each phase
-// loads (with scalar loads) the field bases it uses, instead of all of them
-// staying live in scalar registers across the loop and spilling.
-template <typename R> __device__ __forceinline__ const Args<R> &kargs()
-{
-    uint64_t v = (uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(v));
-    return *(const Args<R> *)(const __attribute__((address_space(4))) Args<R> *)v;
-}
-
 template <typename R, int FL>
 __global__ void __launch_bounds__(TK * TJB<R>) k_fused_main(Args<R> a, int nx, int ny, int nz)
 {
@@ -76,15 +64,15 @@ __global__ void __launch_bounds__(TK * TJB<R>) k_fused_main(Args<R> a, int nx, i
     thread_prologue<R, FL>(T, a, start);
     for (int s = start; s < s1; ++s) {
         const bool wr = s >= s0;
-        phase1<R, FL>(T, L, kargs<R>(), s);
+        phase1<R, FL>(T, L, a, s);
         __syncthreads();
-        phase2<R, FL>(T, L, kargs<R>(), s, wr);
+        phase2<R, FL>(T, L, a, s, wr);
         __syncthreads();
-        phase3<R, FL>(T, L, kargs<R>(), s, wr);
+        phase3<R, FL>(T, L, a, s, wr);
         __syncthreads();
         // phase4 reads only STT/SPP/DT/TP/RT/RP; the next phase1 writes only
         // VR/VT/VP/ETA, so no barrier is needed between them.
-        phase4<R, FL>(T, L, kargs<R>(), s, wr);
+        phase4<R, FL>(T, L, a, s, wr);
     }
     if (FL & F_CHECK) flush_max(a, T.mdv, T.mv, threadIdx.x);
 }
@@ -219,12 +207,17 @@ static int pick_rchunk(int nr, int tiles_xy)
         const int v = std::atoi(e);
         if (v >= 1) return std::min(v, nr);
     }
-    // about 4096 workgroups; a chunk keeps at least 4 planes (each chunk
-    // recomputes 2 warm-up planes), so small boxes still split the r-march
-    // (measured at 63^3 and 127^3: chunks of 4-6 planes are fastest there)
-    int nz = (4096 + tiles_xy - 1) / tiles_xy;
+    // about 3000 workgroups (12 per CU); a chunk keeps at least 4 planes
+    // (each chunk recomputes 2 warm-up planes), so small boxes still split
+    // the r-march (measured at 63^3 and 127^3: chunks of 4-6 planes are
+    // fastest there).  Longer chunks are made odd: at 511^3 chunks of 32 and
+    // 64 planes (concurrent workgroups a power-of-two number of planes apart)
+    // measured 4-6 % slower than 37, 43 or 73.
+    int nz = (3000 + tiles_xy - 1) / tiles_xy;
     nz = std::max(1, std::min(nz, nr / 4));
-    return (nr + nz - 1) / nz;
+    int ch = (nr + nz - 1) / nz;
+    if (ch > 8 && ch % 2 == 0) ++ch;
+    return ch;
 }
 
 // Launches one fused iteration (main kernel + boundary pass) for prepared

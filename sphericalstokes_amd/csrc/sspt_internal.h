@@ -102,7 +102,7 @@ struct ss_ctx {
     bool adj_ready = false;
     int adj_cur = 0;                // lam set holding the current iterate
     double adj_obj_r = 0.0, adj_obj_rad = 0.0; // inclusion of the objective (ss_adjoint_objective)
-    bool adj_fused = true;          // r-marching adjoint kernel (SSPT_ADJ_FUSED=0: the per-family kernels)
+    bool adj_fused = true;          // r-marching adjoint kernel, else the per-family kernels (set by ss_adjoint_prepare)
     bool grid_set = false;
     double *staging = nullptr;      // upload/download staging (largest field)
     size_t staging_elems = 0;

@@ -1,7 +1,7 @@
-"""GPU: the HIP adjoint (gather form, sspt_adjoint.hip) against the CPU
-reference adjoint (scatter form, oracle/sspt_adjoint.c, itself pinned by
-tests/test_adjoint_oracle.py), and its gradients against central
-differences of GPU forward solves.
+"""GPU: the HIP adjoint (gather form, sspt_adjoint.hip: the r-marching kernel
+and the per-family kernels) against the CPU reference adjoint (scatter form,
+oracle/sspt_adjoint.c, itself pinned by tests/test_adjoint_oracle.py), and
+its gradients against central differences of GPU forward solves.
 
 Floating point: the two adjoints sum the same terms in different orders, so
 they agree to rounding (rel. 1e-12 of each field's max); the gradients of
@@ -82,7 +82,9 @@ def test_adjoint_step_matches_reference(Solver, case, path, monkeypatch):
         assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), n
 
 
-def test_adjoint_solve_matches_reference(Solver):
+@pytest.mark.parametrize("path", ["marching", "per_family"])
+def test_adjoint_solve_matches_reference(Solver, path, monkeypatch):
+    monkeypatch.setenv("SSPT_ADJ_FUSED", "1" if path == "marching" else "0")
     st = setups.diapir_sector(9, n_exp=3.0, bc_mode=BC_FREE_SLIP)
     o = Oracle(st)
     o.step(30000)
