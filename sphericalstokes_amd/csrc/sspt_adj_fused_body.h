@@ -21,7 +21,9 @@
 // (phase C): VR(q) is complete then, while Vth, Vph and eta of plane q
 // still wait for the edges of box index q+1 and are finished at step q+1.
 // Values the same column needs at the next plane are carried in registers;
-// values neighbours need go through LDS (two barriers per step).
+// values neighbours need go through LDS (two barriers per step), and so do
+// each thread's accumulators and the previous plane's values that the edges
+// of the next box index need (to keep the register count down).
 //
 // Block = 64 lanes (phi) x ATJ rows (th); lane tx covers k = kb-1+tx, row ty
 // covers j = jb-1+ty; the inner 62 x (ATJ-2) positions are owned.  Outputs
@@ -51,7 +53,7 @@ using fused::NTT;
 using fused::NT2;
 
 #ifndef SSPT_ATJB
-#define SSPT_ATJB 8
+#define SSPT_ATJB 12
 #endif
 constexpr int ATJ = SSPT_ATJB;    // rows per block
 constexpr int AOK = TK - 2;       // owned lanes per row
@@ -154,13 +156,18 @@ SSPT_DH inline bool in_objective(const AArgs &a, int i, int j, int k)
 }
 
 struct ALds {
-    // phase A -> B
-    double wVR[ATJ][TK], wVT[ATJ][TK], wVP[ATJ][TK];
-    double QX[ATJ][TK], QY[ATJ][TK], QZ[ATJ][TK], GSM[ATJ][TK], ETA[ATJ][TK];
+    // phase A -> B; QX, QY, GSM, ETA for planes q and q-1 (index = plane & 1)
+    double wVR[ATJ][TK], wVT[ATJ][TK], wVP[ATJ][TK], QZ[ATJ][TK];
+    double QX[2][ATJ][TK], QY[2][ATJ][TK], GSM[2][ATJ][TK], ETA[2][ATJ][TK];
     // phase B -> C
-    double dT[ATJ][TK], aT[ATJ][TK], dP[ATJ][TK];
+    double tV[ATJ][TK], dP[ATJ][TK]; // centre terms of Vth at j+1 and of Vph at k+1
     double gRT[ATJ][TK], gRP[ATJ][TK], gTP[ATJ][TK], cRT[ATJ][TK], cRP[ATJ][TK], cTP[ATJ][TK];
+    // each thread's own accumulators (Vth, Vph, eta of plane q-1; VR(q)'s
+    // term of centre q-1) and w of its th/phi faces of plane q-1: kept here
+    // rather than in registers
+    double ACC[6][ATJ][TK];
 };
+enum { ACC_VT, ACC_VP, ACC_E, ACC_VRN, ACC_WVT, ACC_WVP };
 
 struct AThread {
     int tx, ty, j, k;
@@ -168,20 +175,14 @@ struct AThread {
     bool own;
     // carries from plane q-1 (or q, loaded ahead)
     double DTq, wVRq, lVRq;          // DT(q), w_VR(q), lam_VR(q)
-    double wVTm, wVPm;               // w_Vth(q-1), w_Vph(q-1)
-    double QXm, QYm;                 // Q sX over (q-1) x (j-1, j); Q sY over (q-1) x (k-1, k)
     double RTq, RTqj, RPq, RPqk;     // tau_Rth at (q, j), (q, j+1); tau_Rph at (q, k), (q, k+1)
     double RTn, RTnj, RPn, RPnk;     // the same of box index q+1 (loaded this step)
-    double gRTm, eRTm, gRPm, eRPm;   // gs SM / eta sums of plane q-1 around the r-th / r-ph edges
-    double accVRn;                   // VR(q): term of centre q-1
-    double accVT, accVP, accE;       // Vth, Vph, eta of plane q-1 without the edges of box index q
     // values of the current step
     double DTq1, lVR1, wVR1, lVT, lVP, wVT, wVP, Q;
     double GS, KB, e, tRR, tTT, tPP, tRT, tRP, tTP;
     double lP, lRR, lTT, lPP, lRT, lRP, lTP;
     double cw_dR, cw_dT, cw_dP, cw_aR, cw_aT, eC;
     double gRT, gRP, gTP, cRT, cRP, cTP;
-    double nQXm, nQYm, ngRTm, neRTm, ngRPm, neRPm;
 };
 
 SSPT_DH inline void athread_init(AThread &T, const AArgs &a, int tx, int ty, int kb, int jb)
@@ -194,14 +195,12 @@ SSPT_DH inline void athread_init(AThread &T, const AArgs &a, int tx, int ty, int
     T.kc = clampi(T.k, 0, a.b.np);
     T.own = tx >= 1 && tx <= TK - 2 && ty >= 1 && ty <= ATJ - 2 && T.j <= a.b.nt && T.k <= a.b.np;
     T.DTq = T.wVRq = T.lVRq = 0.0;
-    T.wVTm = T.wVPm = T.QXm = T.QYm = 0.0;
-    T.gRTm = T.eRTm = T.gRPm = T.eRPm = 0.0;
-    T.accVRn = T.accVT = T.accVP = T.accE = 0.0;
 }
 
 // DT(q), lam_VR(q) and w_VR(q) of the first plane of the march
-SSPT_DH inline void aprologue(AThread &T, const AArgs &a, int q)
+SSPT_DH inline void aprologue(AThread &T, ALds &L, const AArgs &a, int q)
 {
+    for (int m = 0; m < 6; ++m) L.ACC[m][T.ty][T.tx] = 0.0;
     const BoxG &b = a.b;
     const int qc = clampi(q, 0, b.nr);
     T.DTq = fat(a.DT, b.o32(qc, T.jc, T.kc));
@@ -331,11 +330,12 @@ SSPT_DH inline void aphaseA(AThread &T, ALds &L, const AArgs &a, const AStepA &S
     L.wVR[T.ty][T.tx] = T.wVRq;
     L.wVT[T.ty][T.tx] = T.wVT;
     L.wVP[T.ty][T.tx] = T.wVP;
-    L.QX[T.ty][T.tx] = qx;
-    L.QY[T.ty][T.tx] = qy;
+    const int c = q & 1;
+    L.QX[c][T.ty][T.tx] = qx;
+    L.QY[c][T.ty][T.tx] = qy;
     L.QZ[T.ty][T.tx] = qz;
-    L.GSM[T.ty][T.tx] = T.GS;
-    L.ETA[T.ty][T.tx] = T.e;
+    L.GSM[c][T.ty][T.tx] = T.GS;
+    L.ETA[c][T.ty][T.tx] = T.e;
 }
 
 // ---- phase B: adjoints of the new P and tau of centre q and of the edges
@@ -360,6 +360,7 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
     const int jt = clampi(j, 0, nt), jt1 = clampi(j + 1, 0, nt), jtm = clampi(j - 1, 0, nt);
     const uint32_t o = b.o32(qt, T.jc, T.kc);
     const bool up = ty >= 1, lf = tx >= 1, dn = ty + 1 < ATJ, rt_ = tx + 1 < TK;
+    const int c = q & 1, pv = c ^ 1; // LDS planes q and q-1
     const double wR = T.wVRq, wT = T.wVT, wP = T.wVP;
     const double wRj = up ? L.wVR[ty - 1][tx] : 0.0, wRk = lf ? L.wVR[ty][tx - 1] : 0.0;
     const double wTj = dn ? L.wVT[ty + 1][tx] : 0.0, wTk = lf ? L.wVT[ty][tx - 1] : 0.0;
@@ -419,10 +420,10 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
         const double iR = tload(t.tr, qt * NTR + fused::TR_invRf);
         double s = wRj * (a._dth * iR + 0.5 * tload(t.tt, jtm * NTT + fused::TT_cotc2) * iR) +
                    wR * (-a._dth * iR + 0.5 * tload(t.tt, jt * NTT + fused::TT_cotc2) * iR);
-        s += T.wVTm * (a._dr + r15m) + wT * (-a._dr + r15);
-        const double tq = (L.QX[ty][tx] + L.QX[ty - 1][tx]) + T.QXm;
-        const double g4 = ((T.gRTm + L.GSM[ty - 1][tx]) + L.GSM[ty][tx]) * 0.25;
-        const double e4 = ((T.eRTm + L.ETA[ty - 1][tx]) + L.ETA[ty][tx]) * 0.25;
+        s += L.ACC[ACC_WVT][ty][tx] * (a._dr + r15m) + wT * (-a._dr + r15);
+        const double tq = (L.QX[c][ty][tx] + L.QX[c][ty - 1][tx]) + (L.QX[pv][ty][tx] + L.QX[pv][ty - 1][tx]);
+        const double g4 = (((L.GSM[pv][ty - 1][tx] + L.GSM[pv][ty][tx]) + L.GSM[c][ty - 1][tx]) + L.GSM[c][ty][tx]) * 0.25;
+        const double e4 = (((L.ETA[pv][ty - 1][tx] + L.ETA[pv][ty][tx]) + L.ETA[c][ty - 1][tx]) + L.ETA[c][ty][tx]) * 0.25;
         const double v = ((T.lRT + s) + tq) * (1.0 / (1.0 + g4 / e4));
         if (wr && T.own) fat(a.lo[A_RT], o) = v;
         T.gRT = g4 * v;
@@ -430,10 +431,10 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
     }
     if (lf && in_RP(b, q, j, k)) {
         double s = (wRk - wR) * (tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_f) * a._dph);
-        s += T.wVPm * (a._dr + r15m) + wP * (-a._dr + r15);
-        const double tq = (L.QY[ty][tx] + L.QY[ty][tx - 1]) + T.QYm;
-        const double g4 = ((T.gRPm + L.GSM[ty][tx - 1]) + L.GSM[ty][tx]) * 0.25;
-        const double e4 = ((T.eRPm + L.ETA[ty][tx - 1]) + L.ETA[ty][tx]) * 0.25;
+        s += L.ACC[ACC_WVP][ty][tx] * (a._dr + r15m) + wP * (-a._dr + r15);
+        const double tq = (L.QY[c][ty][tx] + L.QY[c][ty][tx - 1]) + (L.QY[pv][ty][tx] + L.QY[pv][ty][tx - 1]);
+        const double g4 = (((L.GSM[pv][ty][tx - 1] + L.GSM[pv][ty][tx]) + L.GSM[c][ty][tx - 1]) + L.GSM[c][ty][tx]) * 0.25;
+        const double e4 = (((L.ETA[pv][ty][tx - 1] + L.ETA[pv][ty][tx]) + L.ETA[c][ty][tx - 1]) + L.ETA[c][ty][tx]) * 0.25;
         const double v = ((T.lRP + s) + tq) * (1.0 / (1.0 + g4 / e4));
         if (wr && T.own) fat(a.lo[A_RP], o) = v;
         T.gRP = g4 * v;
@@ -445,22 +446,14 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
         s += wPj * (a._dth * iR + tload(t.tt, jtm * NTT + fused::TT_cotc2) * iR) +
              wP * (-a._dth * iR + tload(t.tt, jt * NTT + fused::TT_cotc2) * iR);
         const double tq = ((L.QZ[ty][tx] + L.QZ[ty][tx - 1]) + L.QZ[ty - 1][tx]) + L.QZ[ty - 1][tx - 1];
-        const double g4 = (((L.GSM[ty - 1][tx - 1] + L.GSM[ty - 1][tx]) + L.GSM[ty][tx - 1]) + L.GSM[ty][tx]) * 0.25;
-        const double e4 = (((L.ETA[ty - 1][tx - 1] + L.ETA[ty - 1][tx]) + L.ETA[ty][tx - 1]) + L.ETA[ty][tx]) * 0.25;
+        const double g4 = (((L.GSM[c][ty - 1][tx - 1] + L.GSM[c][ty - 1][tx]) + L.GSM[c][ty][tx - 1]) + L.GSM[c][ty][tx]) * 0.25;
+        const double e4 = (((L.ETA[c][ty - 1][tx - 1] + L.ETA[c][ty - 1][tx]) + L.ETA[c][ty][tx - 1]) + L.ETA[c][ty][tx]) * 0.25;
         const double v = ((T.lTP + s) + tq) * (1.0 / (1.0 + g4 / e4));
         if (wr && T.own) fat(a.lo[A_TP], o) = v;
         T.gTP = g4 * v;
         T.cTP = (0.25 * T.tTP * (g4 / (e4 * e4))) * v;
     }
-    // carries of plane q for the edges of box index q+1
-    T.nQXm = up ? L.QX[ty][tx] + L.QX[ty - 1][tx] : 0.0;
-    T.nQYm = lf ? L.QY[ty][tx] + L.QY[ty][tx - 1] : 0.0;
-    T.ngRTm = up ? L.GSM[ty - 1][tx] + L.GSM[ty][tx] : 0.0;
-    T.neRTm = up ? L.ETA[ty - 1][tx] + L.ETA[ty][tx] : 0.0;
-    T.ngRPm = lf ? L.GSM[ty][tx - 1] + L.GSM[ty][tx] : 0.0;
-    T.neRPm = lf ? L.ETA[ty][tx - 1] + L.ETA[ty][tx] : 0.0;
-    L.dT[ty][tx] = T.cw_dT;
-    L.aT[ty][tx] = T.cw_aT;
+    L.tV[ty][tx] = T.cw_dT * (a._dth * tload(t.tr, qt * NTR + fused::TR_invRc)) + 0.5 * T.cw_aT;
     L.dP[ty][tx] = T.cw_dP;
     L.gRT[ty][tx] = T.gRT;
     L.gRP[ty][tx] = T.gRP;
@@ -489,13 +482,13 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
     if (wr_m && T.own && q >= 1) {
         const uint32_t om = b.o32(q - 1, T.jc, T.kc);
         const double mR = -a._dr - 0.5 * iRe;
-        if (j >= 0 && j <= nt && ink) fat(a.lo[A_VT], om) = T.accVT + T.gRT * mR;
-        if (inj && k >= 0 && k <= np) fat(a.lo[A_VP], om) = T.accVP + T.gRP * mR;
-        if (inj && ink) fat(a.lo[A_E], om) = ((T.accE + T.cRT) + cRTj) + (T.cRP + cRPk);
+        if (j >= 0 && j <= nt && ink) fat(a.lo[A_VT], om) = L.ACC[ACC_VT][ty][tx] + T.gRT * mR;
+        if (inj && k >= 0 && k <= np) fat(a.lo[A_VP], om) = L.ACC[ACC_VP][ty][tx] + T.gRP * mR;
+        if (inj && ink) fat(a.lo[A_E], om) = ((L.ACC[ACC_E][ty][tx] + T.cRT) + cRTj) + (T.cRP + cRPk);
     }
     // ---- VR(q): complete
     if (q <= nr && inj && ink) {
-        double v = T.lVRq + T.accVRn;
+        double v = T.lVRq + L.ACC[ACC_VRN][ty][tx];
         if (q <= nr - 1) v += -T.cw_dR * a._dr + 0.5 * T.cw_aR;
         const double dthR = a._dth * iRe;
         v += T.gRT * dthR;
@@ -506,7 +499,7 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
         if (a.bJ && in_objective(a, q, j, k)) v += fat(a.bJ, b.o32(qt, T.jc, T.kc));
         if (wr && T.own) fat(a.lo[A_VR], b.o32(qt, T.jc, T.kc)) = v;
     }
-    T.accVRn = is_cell(b, q, j, k) ? T.cw_dR * a._dr + 0.5 * T.cw_aR : 0.0;
+    L.ACC[ACC_VRN][ty][tx] = is_cell(b, q, j, k) ? T.cw_dR * a._dr + 0.5 * T.cw_aR : 0.0;
     // ---- Vth(q), Vph(q), eta(q) without the edges of box index q+1
     double vT = 0.0, vP = 0.0, vE = 0.0;
     if (q < nr) {
@@ -514,7 +507,7 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
         if (j >= 0 && j <= nt && ink) {
             double v = T.lVT;
             const double dth_R = a._dth * tload(t.tr, qt * NTR + fused::TR_invRc);
-            if (up) v += L.dT[ty - 1][tx] * dth_R + 0.5 * L.aT[ty - 1][tx];
+            if (up) v += L.tV[ty - 1][tx];
             v += -T.cw_dT * dth_R + 0.5 * T.cw_aT;
             v += T.gRT * pR;
             const double r4 = tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_c4) * a._dph;
@@ -539,9 +532,9 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
             vE = ((((T.eC + T.cRT) + T.cRP) + T.cTP) + (cRTj + cRPk)) + ((cTPk + cTPj) + cTPjk);
         }
     }
-    T.accVT = vT;
-    T.accVP = vP;
-    T.accE = vE;
+    L.ACC[ACC_VT][ty][tx] = vT;
+    L.ACC[ACC_VP][ty][tx] = vP;
+    L.ACC[ACC_E][ty][tx] = vE;
     // ---- carries for step q+1
     T.DTq = T.DTq1;
     T.RTq = T.RTn;
@@ -550,14 +543,8 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
     T.RPqk = T.RPnk;
     T.wVRq = T.wVR1;
     T.lVRq = T.lVR1;
-    T.wVTm = T.wVT;
-    T.wVPm = T.wVP;
-    T.QXm = T.nQXm;
-    T.QYm = T.nQYm;
-    T.gRTm = T.ngRTm;
-    T.eRTm = T.neRTm;
-    T.gRPm = T.ngRPm;
-    T.eRPm = T.neRPm;
+    L.ACC[ACC_WVT][ty][tx] = T.wVT;
+    L.ACC[ACC_WVP][ty][tx] = T.wVP;
     (void)up;
 }
 

@@ -669,7 +669,7 @@ __global__ void __launch_bounds__(fused::TK * adjf::ATJ) k_adj_fused(adjf::AArgs
     const int start = max(s0 - 1, 0), stop = min(s1, a.b.nr);
     AThread T;
     athread_init(T, a, threadIdx.x, threadIdx.y, kb, jb);
-    aprologue(T, a, start);
+    aprologue(T, L, a, start);
     AStepA na;
     aloadA(na, a, T, start);
     for (int q = start; q <= stop; ++q) {
@@ -679,9 +679,9 @@ __global__ void __launch_bounds__(fused::TK * adjf::ATJ) k_adj_fused(adjf::AArgs
         aloadB(cb, kargs(), T, q);
         aphaseA(T, L, kargs(), ca, q, wr);
         __syncthreads();
-        aloadA(na, kargs(), T, q + 1); // next step's (clamped into the box at the end)
         aphaseB(T, L, kargs(), cb, q, wr);
         __syncthreads();
+        aloadA(na, kargs(), T, q + 1); // next step's (clamped into the box at the end)
         // phase C reads only the phase-B arrays; the next phase A writes only
         // the phase-A arrays, so no barrier is needed between them
         aphaseC(T, L, kargs(), q, wr, wr_m);
@@ -818,15 +818,16 @@ adjf::AArgs fused_args(ss_ctx *c, const AdjArgs &a, bool with_b)
     return f;
 }
 
-// r-chunk of the marching kernel: about 6 000 workgroups (a few per CU at
-// a time, several rounds) with chunks of at least 4 planes
+// r-chunk of the marching kernel: about 3 000 workgroups (one per CU at a
+// time, 12 rounds) with chunks of at least 4 planes (511^3: chunks of 74
+// planes, 12.9 ms, against 13.1-13.6 ms for 26, 37, 47, 57 or 103)
 int adj_rchunk(int nplanes, int tiles)
 {
     if (const char *e = std::getenv("SSPT_ADJ_RCHUNK")) {
         const int v = std::atoi(e);
         if (v >= 1) return std::min(v, nplanes);
     }
-    int nz = (6144 + tiles - 1) / tiles;
+    int nz = (3000 + tiles - 1) / tiles;
     nz = std::max(1, std::min(nz, nplanes / 4));
     return (nplanes + nz - 1) / nz;
 }

@@ -153,7 +153,7 @@ int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const doub
                     for (int tx = 0; tx < TK; ++tx) {
                         AThread &T = th[ty * TK + tx];
                         athread_init(T, a, tx, ty, kb, jb);
-                        aprologue(T, a, start);
+                        aprologue(T, *L, a, start);
                         aloadA(na[ty * TK + tx], a, T, start);
                     }
                 for (int q = start; q <= stop; ++q) {
