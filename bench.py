@@ -14,7 +14,9 @@ reference CPU path) timed on a bounded sample of the same workload.
 correction sweep on the same 511^3 workload against one FP64 iteration, and
 the time to reach an FP64 residual of 1e-8 with the plain FP64 solve and
 with the FP32-inner / FP64 defect-correction solve (linear diapir_sector at
---ttt-n^3, default 63^3 so that the default run stays within minutes).
+--ttt-n^3, default 127^3: about three minutes of GPU time, where the FP32
+sweeps are already bandwidth-bound; 63^3 is latency-bound and understates
+the gain).
 "adjoint" holds the cost of one adjoint iteration on the same workload
 against one FP64 iteration, and the wall times of a forward solve and of
 the adjoint solve that gives the gradient (power-law diapir at --adjoint-n^3).
@@ -191,7 +193,7 @@ def main():
     ap.add_argument("--n", type=int, default=511)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ttt-n", type=int, default=63,
+    ap.add_argument("--ttt-n", type=int, default=127,
                     help="size of the time-to-tolerance comparison (FP64 vs FP32-refined); 0 skips it")
     ap.add_argument("--ttt-tol", type=float, default=1e-8)
     ap.add_argument("--ttt-tol-inner", type=float, default=1e-5,
