@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -116,18 +117,28 @@ def adjoint_solve_ratio(n, eps):
             "ratio_adjoint_over_forward": round(ta / tf, 4)}
 
 
-def time_to_tolerance(n, tol, tol_inner):
+def time_to_tolerance(n, tol, tol_inner, setup="diapir"):
     """Time-to-tolerance of the plain FP64 solve and of the FP32-inner /
-    FP64 defect-correction solve on the linear diapir_sector at n^3 (free
-    slip: the discrete equations are consistent, so the residual can reach
-    tol).  Both start from the same state; the FP64 solve checks its residual
-    every 500 iterations.  Error of the mixed velocity against the FP64 one."""
+    FP64 defect-correction solve at n^3, free slip (the discrete equations
+    are consistent, so the residual can reach tol): the linear diapir_sector
+    (setup "diapir") or the power-law diapir with the smoothed log-normal
+    matrix viscosity (setup "lognormal", the mixed-precision configuration;
+    the outer steps are then also Picard steps of the viscosity).  Both start
+    from the same state; the FP64 solve checks its residual every 500
+    iterations.  After every outer step the error of V (max over the
+    components) and of the interior zero-mean P against the FP64 solution
+    is measured (downloads outside the timed calls)."""
     import numpy as np
     from sphericalstokes_amd import setups
     from sphericalstokes_amd.fields import BC_FREE_SLIP, FIELDS
     from sphericalstokes_amd.solver import Solver
 
-    st = setups.diapir_sector(n, n_exp=1.0, bc_mode=BC_FREE_SLIP)
+    if setup == "lognormal":
+        st = setups.diapir_lognormal(n)
+        what = f"diapir_lognormal {n}^3 (n_exp=3, log10 eta_matrix ~ N(0, 0.5^2) seed 0, smoothed 8 cells, contrast <= 1e4)"
+    else:
+        st = setups.diapir_sector(n, n_exp=1.0, bc_mode=BC_FREE_SLIP)
+        what = f"diapir_sector {n}^3, n_exp=1"
     p = st.params
 
     def load():
@@ -149,11 +160,19 @@ def time_to_tolerance(n, tol, tol_inner):
         conv64 = r64["converged"]
         print(f"  fp64 {n}^3: {iters} iterations, residual {r64['r']:.3e}", file=sys.stderr, flush=True)
     t64 = time.perf_counter() - t0
-    ref = {k: s.download(k) for k in ("VR", "Vth", "Vph")}
+    ref = {k: s.download(k) for k in ("VR", "Vth", "Vph", "P")}
     s.close()
+
+    def errors(s):
+        ev = max(float(np.linalg.norm(s.download(k) - ref[k]) / np.linalg.norm(ref[k])) for k in ("VR", "Vth", "Vph"))
+        dp = (s.download("P") - ref["P"])[1:-1, 1:-1, 1:-1]
+        pc = ref["P"][1:-1, 1:-1, 1:-1]
+        ep = float(np.linalg.norm(dp - dp.mean()) / np.linalg.norm(pc - pc.mean()))
+        return ev, ep
+
     s = load()
-    t0 = time.perf_counter()
-    outer, inner, hist, convm = 0, 0, [], False
+    tm = 0.0
+    outer, inner, hist, convm, ehist = 0, 0, [], False, []
     tin_prev, r_prev = tol_inner, None
     while outer < 60 and not convm:
         # one outer step per call (progress on stderr); the inner tolerance
@@ -166,23 +185,78 @@ def time_to_tolerance(n, tol, tol_inner):
         tin_prev = tin
         # an outer step stops after 100000 sweeps at most (a partial correction is
         # still a valid defect-correction step; this bounds the time per call)
+        t0 = time.perf_counter()
         rm = s.pt_solve_mixed(p, max_outer=1, max_inner=100_000, nout_inner=200, tol_inner=tin, tol=tol)
+        tm += time.perf_counter() - t0
+        ehist.append(errors(s))
         if not hist:
             hist.append(rm["r_hist"][0])
         convm = rm["converged"]
         outer += rm["outer"]
         inner += rm["inner_total"]
         hist.append(rm["r"])
-        print(f"  mixed {n}^3: outer {outer}, {inner} FP32 sweeps, residual {rm['r']:.3e}", file=sys.stderr,
-              flush=True)
-    tm = time.perf_counter() - t0
-    err = max(float(np.linalg.norm(s.download(k) - ref[k]) / np.linalg.norm(ref[k])) for k in ref)
+        print(f"  mixed {n}^3: outer {outer}, {inner} FP32 sweeps, residual {rm['r']:.3e}, "
+              f"error vs FP64: V {ehist[-1][0]:.2e} P {ehist[-1][1]:.2e}", file=sys.stderr, flush=True)
     s.close()
-    return {"workload": f"diapir_sector {n}^3, n_exp=1, free slip, FP64 residual <= {tol:g}",
+    return {"workload": f"{what}, free slip, FP64 residual <= {tol:g}",
             "fp64_s": round(t64, 3), "fp64_iters": iters, "fp64_converged": conv64,
             "mixed_s": round(tm, 3), "mixed_outer": outer, "mixed_inner_sweeps": inner,
             "mixed_converged": convm, "mixed_r_hist": [float("%.3e" % x) for x in hist],
-            "ratio_mixed_over_fp64": round(tm / t64, 4), "V_rel_err_mixed_vs_fp64": float("%.3e" % err)}
+            "mixed_V_err_hist": [float("%.3e" % e[0]) for e in ehist],
+            "mixed_P_err_hist": [float("%.3e" % e[1]) for e in ehist],
+            "ratio_mixed_over_fp64": round(tm / t64, 4), "V_rel_err_mixed_vs_fp64": float("%.3e" % ehist[-1][0]),
+            "P_rel_err_mixed_vs_fp64": float("%.3e" % ehist[-1][1])}
+
+
+def mixed_large(n, outer, sweeps):
+    """The mixed-precision configuration at full size: the power-law diapir
+    with the log-normal matrix viscosity at n^3 (767^3: 31 FP64 and 25 FP32
+    boxes, about 157 GB of HBM).  Times FP64 iterations and runs `outer`
+    outer steps of `sweeps` FP32 sweeps each, reporting the FP64 residual
+    after every step (a solve to 1e-8 at this size takes hours).  With this
+    viscosity field the PT iteration (the reference's step-size rule, even
+    at 0.7 of its CFL) is unstable beyond about 127^3 cells; the timings are
+    still those of the workload, and a non-finite residual is reported as
+    null with the solver's message."""
+    from sphericalstokes_amd import setups
+    from sphericalstokes_amd._lib import SSError
+    from sphericalstokes_amd.fields import FIELDS
+    from sphericalstokes_amd.solver import Solver
+
+    t0 = time.perf_counter()
+    cfl_factor = 0.7           # the reference's CFL rule diverges beyond ~63^3 at this contrast
+    st = setups.diapir_lognormal(n, cfl_factor=cfl_factor)
+    print(f"  mixed {n}^3: setup built in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    p = st.params
+    s = Solver(n, n, n)
+    s.set_grid(st.R, st.th, st.ph)
+    for name in FIELDS:
+        s.upload(name, st.fields[name])
+    del st.fields
+    print(f"  mixed {n}^3: uploaded", file=sys.stderr, flush=True)
+    s.pt_iterate(p, 3)
+    main_ms, it_ms = s.pt_iterate_timed(p, 10)
+    hist, tin, done, note = [], 0.0, 0, None
+    for k in range(outer):
+        try:
+            r = s.pt_solve_mixed(p, max_outer=1, max_inner=sweeps, nout_inner=sweeps + 1, tol_inner=0.0, tol=0.0)
+        except SSError as e:
+            note = str(e)
+            break
+        if not hist:
+            hist.append(r["r_hist"][0])
+        hist.append(r["r"])
+        tin += r["inner_seconds"]
+        done += 1
+        print(f"  mixed {n}^3: outer {k + 1}, residual {r['r']:.3e}", file=sys.stderr, flush=True)
+    s.close()
+    sweep_ms = tin / (done * sweeps) * 1e3 if done else None
+    return {"workload": f"diapir_lognormal {n}^3 (n_exp=3, log10 eta_matrix ~ N(0, 0.5^2) seed 0, smoothed 8 cells, "
+                        f"contrast <= 1e4, CFL x {cfl_factor}), free slip", "fp64_iter_ms": round(it_ms, 4),
+            "fp64_kernel_ms": round(main_ms, 4), "fp32_sweep_ms": round(sweep_ms, 4) if sweep_ms else None,
+            "sweep_over_iter": round(sweep_ms / it_ms, 4) if sweep_ms else None,
+            "outer_steps": done, "sweeps_per_outer": sweeps,
+            "r_hist": [float("%.3e" % x) if math.isfinite(x) else None for x in hist], "error": note}
 
 
 def main():
@@ -196,6 +270,12 @@ def main():
     ap.add_argument("--ttt-n", type=int, default=127,
                     help="size of the time-to-tolerance comparison (FP64 vs FP32-refined); 0 skips it")
     ap.add_argument("--ttt-tol", type=float, default=1e-8)
+    ap.add_argument("--mixed-large-n", type=int, default=0,
+                    help="also run outer steps of the mixed solve on the log-normal power-law diapir at this "
+                         "size (767 for the full configuration); 0 skips it")
+    ap.add_argument("--ttt-setup", choices=["diapir", "lognormal"], default="diapir",
+                    help="time-to-tolerance workload: the linear diapir_sector, or the power-law diapir "
+                         "with the smoothed log-normal matrix viscosity")
     ap.add_argument("--ttt-tol-inner", type=float, default=1e-5,
                     help="inner tolerance of the FP32 sweeps (max relative change per sweep)")
     ap.add_argument("--fp32-sweeps", type=int, default=20,
@@ -269,7 +349,10 @@ def main():
                       "iter_over_fp64_iter": round(adj_ms / ms_per_step, 4) if adj_ms else None,
                       "solve": adjoint_solve_ratio(args.adjoint_n, 1e-10) if args.adjoint_n > 0 else None}
     if args.ttt_n > 0:
-        out["mixed_precision"]["time_to_tol"] = time_to_tolerance(args.ttt_n, args.ttt_tol, args.ttt_tol_inner)
+        out["mixed_precision"]["time_to_tol"] = time_to_tolerance(args.ttt_n, args.ttt_tol, args.ttt_tol_inner,
+                                                                  args.ttt_setup)
+    if args.mixed_large_n > 0:
+        out["mixed_precision"]["large"] = mixed_large(args.mixed_large_n, 3, 1000)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(st, n, args.cpu_iters)
     print(json.dumps(out), flush=True)
