@@ -45,6 +45,7 @@ class orc_adj(C.Structure):
 
 
 _LIB = None
+_DEFAULT_THREADS = None
 
 
 def lib():
@@ -72,6 +73,7 @@ def lib():
             "orc_pt_step": (None, [SP, PP, I]),
             "orc_pt_solve": (I, [SP, PP, I, I, D, I, DP, C.POINTER(I)]),
             "orc_num_threads": (I, []),
+            "orc_set_num_threads": (None, [I]),
             "orc_adj_step": (None, [SP, PP, C.POINTER(orc_adj), DP, DP]),
             "orc_adj_objective": (D, [SP, D, D, DP]),
             "orc_adj_solve": (I, [SP, PP, D, D, C.POINTER(orc_adj), I, I, D, DP, DP]),
@@ -102,6 +104,13 @@ class Oracle:
 
     def __init__(self, setup, fields=None, copy=True):
         self.L = lib()
+        # grids below 24^3 cells run on one thread (the parallel loops' fork
+        # and join would dominate), larger ones on all
+        global _DEFAULT_THREADS
+        if _DEFAULT_THREADS is None:
+            _DEFAULT_THREADS = self.L.orc_num_threads()
+        small = setup.nr * setup.nth * setup.nph < 24 ** 3
+        self.L.orc_set_num_threads(1 if small else _DEFAULT_THREADS)
         self.setup = setup
         src = fields if fields is not None else setup.fields
         if copy:

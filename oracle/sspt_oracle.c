@@ -47,6 +47,16 @@ int orc_num_threads(void)
 #endif
 }
 
+/* threads of the parallel loops from now on (tiny grids run faster on one) */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n >= 1) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* SphericalStokes_diapir.jl:46-58 */
 void orc_equal3(double *VR_err, double *VT_err, double *VP_err,
                 const double *VR, const double *VT, const double *VP,

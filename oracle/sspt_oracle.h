@@ -81,6 +81,7 @@ int orc_pt_solve(orc_state *s, const orc_params *p, int maxiter, int nout,
                  double eps, int always_equal3, double *err_out, int *iter_end);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

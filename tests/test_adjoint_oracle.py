@@ -85,18 +85,23 @@ def test_adjoint_gradient_matches_central_differences(case):
     base = {n: o.f[n].copy(order="F") for n in o.f}
     rng = np.random.default_rng(1)
     h = 1e-5
+    iters_fd = 14000
+    # 16 Gaussian directions per parameter from seed 1 (the adjoint
+    # configuration's check); each perturbed solve starts from the converged
+    # state
     for what in ("rho", "log10_eta_ini"):
-        dm = rng.standard_normal(base["rho"].shape)
-        J = []
-        for sgn in (+1, -1):
-            f = {n: base[n].copy(order="F") for n in base}
-            if what == "rho":
-                f["rho"] = np.asfortranarray(f["rho"] + sgn * h * dm)
-            else:
-                f["eta_ini"] = np.asfortranarray(f["eta_ini"] * 10.0 ** (sgn * h * dm))
-            ob = Oracle(st, f)
-            ob.step(iters)
-            J.append(ob.objective(R0, RAD)[0])
-        fd = (J[0] - J[1]) / (2 * h)
-        ad = float(np.sum((g_rho if what == "rho" else g_log) * dm))
-        assert abs(fd - ad) <= 1e-6 * abs(fd), (what, fd, ad)
+        for d in range(16):
+            dm = rng.standard_normal(base["rho"].shape)
+            J = []
+            for sgn in (+1, -1):
+                f = {n: base[n].copy(order="F") for n in base}
+                if what == "rho":
+                    f["rho"] = np.asfortranarray(f["rho"] + sgn * h * dm)
+                else:
+                    f["eta_ini"] = np.asfortranarray(f["eta_ini"] * 10.0 ** (sgn * h * dm))
+                ob = Oracle(st, f)
+                ob.step(iters_fd)   # from the converged state: the 1e-5 change converges in fewer steps
+                J.append(ob.objective(R0, RAD)[0])
+            fd = (J[0] - J[1]) / (2 * h)
+            ad = float(np.sum((g_rho if what == "rho" else g_log) * dm))
+            assert abs(fd - ad) <= 1e-6 * abs(fd), (what, d, fd, ad)

@@ -105,7 +105,9 @@ def test_adjoint_solve_matches_reference(Solver, path, monkeypatch):
 @pytest.mark.parametrize("n_exp", [1.0, 3.0])
 def test_adjoint_gradient_central_differences(Solver, n_exp):
     """Gradient of J at 17^3 on the GPU against central differences (h = 1e-6)
-    of GPU forward solves along seeded random directions."""
+    of GPU forward solves along 4 Gaussian directions per parameter from
+    seed 1 (16 per parameter run on the CPU reference adjoint at 7^3,
+    tests/test_adjoint_oracle.py; this test stays under a minute)."""
     st = setups.diapir_sector(17, n_exp=n_exp, bc_mode=BC_FREE_SLIP, vr=10.0)
     p = st.params
     iters = 60000
@@ -117,23 +119,26 @@ def test_adjoint_gradient_central_differences(Solver, n_exp):
     assert r["converged"]
     rng = np.random.default_rng(1)
     h = 1e-6
+    worst = 0.0
     for what in ("rho", "log10_eta_ini"):
-        dm = rng.standard_normal(base["rho"].shape)
-        J = []
-        for sgn in (+1, -1):
-            f = {k: v.copy(order="F") for k, v in base.items()}
-            if what == "rho":
-                f["rho"] = np.asfortranarray(f["rho"] + sgn * h * dm)
-            else:
-                f["eta_ini"] = np.asfortranarray(f["eta_ini"] * 10.0 ** (sgn * h * dm))
-            with _load(Solver, st, f) as s2:
-                s2.pt_iterate(p, iters)
-                s2.adjoint_prepare(p)
-                J.append(s2.adjoint_objective(R0, RAD))
-        fd = (J[0] - J[1]) / (2 * h)
-        ad = float(np.sum((r["grad_rho"] if what == "rho" else r["grad_log10_eta_ini"]) * dm))
-        say(what, "fd", fd, "adjoint", ad)
-        assert abs(fd - ad) <= 1e-5 * abs(fd), (what, fd, ad)
+        for d in range(4):
+            dm = rng.standard_normal(base["rho"].shape)
+            J = []
+            for sgn in (+1, -1):
+                f = {k: v.copy(order="F") for k, v in base.items()}
+                if what == "rho":
+                    f["rho"] = np.asfortranarray(f["rho"] + sgn * h * dm)
+                else:
+                    f["eta_ini"] = np.asfortranarray(f["eta_ini"] * 10.0 ** (sgn * h * dm))
+                with _load(Solver, st, f) as s2:
+                    s2.pt_iterate(p, iters)
+                    s2.adjoint_prepare(p)
+                    J.append(s2.adjoint_objective(R0, RAD))
+            fd = (J[0] - J[1]) / (2 * h)
+            ad = float(np.sum((r["grad_rho"] if what == "rho" else r["grad_log10_eta_ini"]) * dm))
+            worst = max(worst, abs(fd - ad) / abs(fd))
+            assert abs(fd - ad) <= 1e-5 * abs(fd), (what, d, fd, ad)
+    say("worst relative difference over 8 directions", worst)
 
 
 def test_autograd_function_matches_central_differences():
