@@ -51,11 +51,13 @@ def _read(path: Path, shp) -> np.ndarray:
 
 def save_reference_format(out, source, setup, *, num: int = 1, eps: float = 1e-8, runtime: float = 0.0,
                           err_evo=(), iter_evo=(), eta0: float = 1.0, rho0: float = 0.0, rho_in: float = -1.0,
-                          vr: float = 1e3, radius: float = 0.01) -> Path:
+                          vr: float = 1e3, radius: float = 0.01, phys_line: str | None = None) -> Path:
     """Writes the reference's output set for `setup` (grid, parameters) with
     the fields of `source`: a Solver (downloaded) or a dict name -> array.
-    The physical constants of the _phys.inf line that the solver does not
-    hold (eta0, rho0, rho_in, vr, radius) are taken from the keywords."""
+    The physical constants of the diapir's _phys.inf line that the solver
+    does not hold (eta0, rho0, rho_in, vr, radius) are taken from the
+    keywords; phys_line replaces that line (the plateau script writes its
+    own)."""
     out = Path(out)
     out.mkdir(parents=True, exist_ok=True)
     p = setup.params
@@ -69,6 +71,7 @@ def save_reference_format(out, source, setup, *, num: int = 1, eps: float = 1e-8
     (out / f"{num}_infos.inf").write_text(
         "%d %f %f %f %d %d %d %d %d" % (num, lr, lth, lph, nr, nth, nph, round(eps), round(runtime)))
     (out / f"{num}_phys.inf").write_text(
+        phys_line if phys_line is not None else
         "%d   %f  %f  %f    %f     %f   %f  %f, %f, %f, %f    , %f"
         % (num, eta0, rho0, rho_in, p.n_exp, p.tau_C, vr, p.g, p.sh, p.r, radius, p.dmp))
     _write(out / "A_err_evo.bin", np.asarray(err_evo, dtype=np.float64))
