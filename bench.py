@@ -208,6 +208,40 @@ def time_to_tolerance(n, tol, tol_inner, setup="diapir"):
             "P_rel_err_mixed_vs_fp64": float("%.3e" % ehist[-1][1])}
 
 
+def hbm_measured():
+    """Streaming rates this GPU reaches on plain torch kernels (1 GiB FP64
+    arrays, 20 repetitions timed with events): copy (1 read + 1 write),
+    triad c = a + 0.5 b (2 reads + 1 write), read-only sum, fill.  They
+    place the stencil kernel's rate between the vendor peak and what simple
+    streaming reaches."""
+    import torch
+    dev = torch.device("cuda:0")
+    n = (1 << 30) // 8
+    a = torch.rand(n, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    c = torch.rand(n, dtype=torch.float64, device=dev)
+
+    def rate(f, nbytes, reps=20):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return round(nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9, 1)
+
+    B = n * 8
+    out = {"copy_GBps": rate(lambda: b.copy_(a), 2 * B),
+           "triad_GBps": rate(lambda: torch.add(a, c, alpha=0.5, out=b), 3 * B),
+           "read_GBps": rate(lambda: a.sum(), B),
+           "write_GBps": rate(lambda: b.fill_(1.0), B)}
+    del a, b, c
+    torch.cuda.empty_cache()
+    return out
+
+
 def mixed_large(n, outer, sweeps):
     """The mixed-precision configuration at full size: the power-law diapir
     with the log-normal matrix viscosity at n^3 (767^3: 31 FP64 and 25 FP32
@@ -270,6 +304,8 @@ def main():
     ap.add_argument("--ttt-n", type=int, default=127,
                     help="size of the time-to-tolerance comparison (FP64 vs FP32-refined); 0 skips it")
     ap.add_argument("--ttt-tol", type=float, default=1e-8)
+    ap.add_argument("--no-hbm-probe", action="store_true",
+                    help="skip the plain streaming-rate measurement (copy/triad/read/write)")
     ap.add_argument("--mixed-large-n", type=int, default=0,
                     help="also run outer steps of the mixed solve on the log-normal power-law diapir at this "
                          "size (767 for the full configuration); 0 skips it")
@@ -351,6 +387,8 @@ def main():
     if args.ttt_n > 0:
         out["mixed_precision"]["time_to_tol"] = time_to_tolerance(args.ttt_n, args.ttt_tol, args.ttt_tol_inner,
                                                                   args.ttt_setup)
+    if not args.no_hbm_probe:
+        out["hbm_measured"] = hbm_measured()
     if args.mixed_large_n > 0:
         out["mixed_precision"]["large"] = mixed_large(args.mixed_large_n, 3, 1000)
     if not args.no_cpu:
