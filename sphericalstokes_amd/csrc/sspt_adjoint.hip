@@ -818,18 +818,16 @@ adjf::AArgs fused_args(ss_ctx *c, const AdjArgs &a, bool with_b)
     return f;
 }
 
-// r-chunk of the marching kernel: about 3 000 workgroups (one per CU at a
-// time, 12 rounds) with chunks of at least 4 planes (511^3: chunks of 74
-// planes, 12.9 ms, against 13.1-13.6 ms for 26, 37, 47, 57 or 103)
-int adj_rchunk(int nplanes, int tiles)
+// r-chunk length of the marching kernel (march_chunk; 511^3: chunks of 74
+// planes, 12.9 ms, against 13.1-13.6 ms for 26, 37, 47, 57 or 103);
+// SSPT_ADJ_RCHUNK overrides it for measurements.
+int adj_rchunk(const ss_ctx *c, int nplanes, int tiles)
 {
     if (const char *e = std::getenv("SSPT_ADJ_RCHUNK")) {
         const int v = std::atoi(e);
         if (v >= 1) return std::min(v, nplanes);
     }
-    int nz = (3000 + tiles - 1) / tiles;
-    nz = std::max(1, std::min(nz, nplanes / 4));
-    return (nplanes + nz - 1) / nz;
+    return march_chunk(nplanes, tiles, c->n_cu, false);
 }
 
 // lam_out = T^T lam_in (+ bJ on VR if given); gradients when a.grad
@@ -845,7 +843,7 @@ void launch_adj_step(ss_ctx *c, AdjArgs a, bool with_b)
         adjf::AArgs f = fused_args(c, a, with_b);
         hipLaunchKernelGGL(k_adj_lb, dim3((m + 63) / 64, (m + 3) / 4, 3), B2, 0, c->stream, f, c->adj[AD_LB]);
         const int nx = (b.np + 1 + adjf::AOK - 1) / adjf::AOK, ny = (b.nt + 1 + adjf::AOJ - 1) / adjf::AOJ;
-        f.rchunk = adj_rchunk(b.nr + 1, nx * ny);
+        f.rchunk = adj_rchunk(c, b.nr + 1, nx * ny);
         const int nz = (b.nr + 1 + f.rchunk - 1) / f.rchunk;
         const int T = nx * ny * nz;
         hipLaunchKernelGGL(k_adj_fused, dim3(((T + 7) / 8) * 8), dim3(fused::TK, adjf::ATJ), 0, c->stream, f, nx, ny,

@@ -171,6 +171,11 @@ int ss_create(int nr, int nth, int nph, int device, ss_ctx **out)
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) { delete c; return (int)e; }
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+            c->n_cu = ncu;
+    }
     Box &b = c->box;
     b.nr = nr; b.nt = nth; b.np = nph;
     b.pitch = ((nph + 1) + 63) / 64 * 64;

@@ -199,27 +199,14 @@ static Args<double> make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out
     return a;
 }
 
-// r-chunking: enough workgroups to fill 256 CUs several times over while
-// keeping the two warm-up planes of each chunk a small fraction.
-static int pick_rchunk(int nr, int tiles_xy, bool fp32)
+// r-chunk length (march_chunk); SSPT_RCHUNK overrides it for measurements.
+static int pick_rchunk(const ss_ctx *c, int nr, int tiles_xy, bool fp32)
 {
     if (const char *e = std::getenv("SSPT_RCHUNK")) {
         const int v = std::atoi(e);
         if (v >= 1) return std::min(v, nr);
     }
-    // FP64: about 3000 workgroups (12 per CU); a chunk keeps at least 4
-    // planes (each chunk recomputes 2 warm-up planes), so small boxes still
-    // split the r-march (measured at 63^3 and 127^3: chunks of 4-6 planes are
-    // fastest there).  Longer chunks are made odd: at 511^3 chunks of 32 and
-    // 64 planes (concurrent workgroups a power-of-two number of planes apart)
-    // measured 4-6 % slower than 37, 43 or 73.  FP32 sweeps: about 4096
-    // workgroups (511^3: chunks of 40 planes, 3.71 ms against 3.8-3.9 ms for
-    // 37, 53 or 73).
-    int nz = ((fp32 ? 4096 : 3000) + tiles_xy - 1) / tiles_xy;
-    nz = std::max(1, std::min(nz, nr / 4));
-    int ch = (nr + nz - 1) / nz;
-    if (!fp32 && ch > 8 && ch % 2 == 0) ++ch;
-    return ch;
+    return march_chunk(nr, tiles_xy, c->n_cu, !fp32);
 }
 
 // Launches one fused iteration (main kernel + boundary pass) for prepared
@@ -230,7 +217,7 @@ void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_
     a.pp_in = a.pp_out = a.ppe_in = a.ppe_out = 0;
     const int nx = (a.b.np + OK_ - 1) / OK_;
     const int ny = (a.b.nt + OJ<R> - 1) / OJ<R>;
-    a.rchunk = pick_rchunk(a.b.nr, nx * ny, sizeof(R) == 4);
+    a.rchunk = pick_rchunk(c, a.b.nr, nx * ny, sizeof(R) == 4);
     const int nz = (a.b.nr + a.rchunk - 1) / a.rchunk;
     if (a.flags & F_CHECK) hipLaunchKernelGGL(k_zero_red, dim3(1), dim3(64), 0, c->stream, a.red);
     const int T = nx * ny * nz;

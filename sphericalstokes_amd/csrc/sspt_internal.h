@@ -75,10 +75,41 @@ enum { SS_F32_ETA = 20, SS_F32_GP = 21, SS_F32_FVR = 22, SS_F32_FVT = 23, SS_F32
 // FP64 boxes of the adjoint solve (sspt_adjoint.hip lists them)
 enum { SS_ADJ_NBOX = 50 };
 
+// Length of the r-chunks of a marching kernel (grid z).  A workgroup marches
+// its chunk plus two warm-up planes, and one workgroup fits a CU at a time,
+// so a launch over `tiles` column tiles costs about
+//     ceil(tiles * ceil(nplanes / ch) / slots) * (ch + 2)
+// plane steps of one CU.  The chunk minimising that count is taken (ties to
+// the shorter chunk), between 3 and 80 planes: longer chunks measured slower
+// at 383^3-767^3 whatever the count says (767^3 FP64: 59 planes 21.6 ms,
+// 257 planes 24.0 ms, 767 planes 24.9 ms), and the count picks the measured
+// best below (FP64 63^3: 4, 127^3: 23, 191^3: 65, 255^3: 17, 383^3: 35,
+// 511^3: 73; FP32 127^3: 16, 767^3: 77).  odd_long keeps chunks above 8
+// planes odd (FP64 at 511^3: chunks of 32 and 64 planes, concurrent
+// workgroups a power-of-two number of planes apart, measured 4-6 % slower).
+inline int march_chunk(int nplanes, int tiles, int slots, bool odd_long)
+{
+    int best = nplanes < 3 ? nplanes : 3;
+    long long best_cost = -1, best_wg = 0;
+    const int top = nplanes < 80 ? nplanes : 80;
+    for (int ch = 3; ch <= top; ++ch) {
+        if (odd_long && ch > 8 && ch % 2 == 0) continue;
+        const long long wg = (long long)tiles * ((nplanes + ch - 1) / ch);
+        const long long cost = ((wg + slots - 1) / slots) * (ch + 2);
+        if (best_cost < 0 || cost < best_cost || (cost == best_cost && wg > best_wg)) {
+            best = ch;
+            best_cost = cost;
+            best_wg = wg;
+        }
+    }
+    return best;
+}
+
 } // namespace sspt
 
 struct ss_ctx {
     int device = 0;
+    int n_cu = 256;                 // compute units of the device (workgroup slots of the marching kernels)
     sspt::Box box{};
     size_t box_elems = 0;
     hipStream_t stream = nullptr;
