@@ -209,6 +209,24 @@ def time_to_tolerance(n, tol, tol_inner, setup="diapir"):
 
 
 def hbm_measured():
+    """Streaming rates of plain torch kernels, measured in a child process.
+
+    torch brings its own HIP runtime; once libsspt.so (linked against the
+    system one) holds the GPU in this process, torch's runtime finds no
+    device.  The child process has only torch's runtime.  Returns the rates,
+    or {"error": ...} if the child fails."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--hbm-probe-child"],
+                           capture_output=True, text=True, timeout=300)
+    except subprocess.TimeoutExpired:
+        return {"error": "streaming-rate probe timed out"}
+    if r.returncode != 0:
+        return {"error": f"streaming-rate probe exited with {r.returncode}: {r.stderr.strip()[-300:]}"}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def hbm_probe():
     """Streaming rates this GPU reaches on plain torch kernels (1 GiB FP64
     arrays, 20 repetitions timed with events): copy (1 read + 1 write),
     triad c = a + 0.5 b (2 reads + 1 write), read-only sum, fill.  They
@@ -323,7 +341,11 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain field copy (k_copy_field: n^3 doubles read + written) so that "
                          "rocprofv3 --pmc byte counters can be scaled")
+    ap.add_argument("--hbm-probe-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.hbm_probe_child:        # the child process of hbm_measured()
+        print(json.dumps(hbm_probe()), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the PT iteration runs one rank's block on one MI355X; "
               "multi-GPU halo exchange is not part of this build.", file=sys.stderr)

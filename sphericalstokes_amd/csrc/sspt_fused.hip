@@ -49,8 +49,16 @@ __device__ __forceinline__ void tile_of(int nx, int ny, int nz, int &bx, int &by
     bz = t / (nx * ny);
 }
 
+// SSPT_MAIN_WAVES64: optional waves-per-SIMD floor for the FP64 instantiations
+// (register budget 512 / waves), for tile-shape measurements.
+#ifdef SSPT_MAIN_WAVES64
+#define SSPT_MAIN_ATTR __attribute__((amdgpu_waves_per_eu(SSPT_MAIN_WAVES64)))
+#else
+#define SSPT_MAIN_ATTR
+#endif
+
 template <typename R, int FL>
-__global__ void __launch_bounds__(TK * TJB<R>) k_fused_main(Args<R> a, int nx, int ny, int nz)
+__global__ void __launch_bounds__(TK * TJB<R>) SSPT_MAIN_ATTR k_fused_main(Args<R> a, int nx, int ny, int nz)
 {
     __shared__ Lds<R> L;
     int bx, by, bz;
