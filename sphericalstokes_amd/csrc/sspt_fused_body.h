@@ -193,6 +193,16 @@ template <typename R, int FL> SSPT_DH inline R &wout(R *out, const R *in, uint32
     else
         return fat(out, o);
 }
+// Stores a new state value (SSPT_NT_STORE: with the non-temporal hint, for
+// measurements of the cache policy of the output stream).
+template <typename R, int FL> SSPT_DH inline void st_out(R *out, const R *in, uint32_t o, uint32_t pp, R v)
+{
+#if defined(SSPT_NT_STORE) && defined(__HIP_DEVICE_COMPILE__)
+    __builtin_nontemporal_store(v, &wout<R, FL>(out, in, o, pp));
+#else
+    wout<R, FL>(out, in, o, pp) = v;
+#endif
+}
 
 template <typename R> struct Lds {
     R VR[TJB<R>][TK], VT[TJB<R>][TK], VP[TJB<R>][TK];
@@ -480,10 +490,10 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         T.nSPP = -nP + T.nPP;
         if (wr && T.own) {
             const uint32_t o = b.o32(s, T.j, T.k);
-            wout<R, FL>(a.oP, a.P, o, a.pp_out) = nP;
-            wout<R, FL>(a.oRR, a.tRR, o, a.pp_out) = T.nRR;
-            wout<R, FL>(a.oTT, a.tTT, o, a.pp_out) = T.nTT;
-            wout<R, FL>(a.oPP, a.tPP, o, a.pp_out) = T.nPP;
+            st_out<R, FL>(a.oP, a.P, o, a.pp_out, nP);
+            st_out<R, FL>(a.oRR, a.tRR, o, a.pp_out, T.nRR);
+            st_out<R, FL>(a.oTT, a.tTT, o, a.pp_out, T.nTT);
+            st_out<R, FL>(a.oPP, a.tPP, o, a.pp_out, T.nPP);
             if ((FL & F_LAST) && !(FL & F_CORR)) {
                 fat(a.DT, o) = T.nDT;
                 fat(a.SM, o) = T.sm0;
@@ -507,7 +517,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         const R D = ((tload(t.tr, s * NTR + TR_invRc4) * dVP) * a._dth + (tload(t.t2, (s * (nt + 1) + T.j) * NT2 + T2_c4) * dVT) * a._dph) -
                          div_rcp(avVP, tload(t.tr, s * NTR + TR_Rc4), tload(t.tr, s * NTR + TR_invRc4)) * tload(t.tt, T.j * NTT + TT_cotf4);
         T.ntp = (T.oTPold + T.gtp * D) / (R(1) + T.gtp / T.etp);
-        if (wr && T.own) wout<R, FL>(a.oTP, a.tTP, b.o32(s, T.j, T.k), a.pp_out) = T.ntp;
+        if (wr && T.own) st_out<R, FL>(a.oTP, a.tTP, b.o32(s, T.j, T.k), a.pp_out, T.ntp);
     }
     L.TP[T.ty][T.tx] = T.ntp;
 
@@ -533,7 +543,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         const R dt = (T.cDT + T.nDT) * R(0.5);
         const R v = T.vr0 + dV * dt;
         const uint32_t o = b.o32(s, T.j, T.k);
-        wout<R, FL>(a.oVR, a.VR, o, a.pp_out) = v;
+        st_out<R, FL>(a.oVR, a.VR, o, a.pp_out, v);
         if (FL & F_CHECK) {
             const R d = v - T.vr0;
             if (!(FL & F_CORR)) fat(a.VRe, o) = d;
@@ -587,7 +597,7 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R D = ((T.vt1 - T.vt0) * a._dr + (iRe * (T.vr1 - L.VR[T.ty - 1][T.tx])) * a._dth) -
                              div_rcp((T.vt0 + T.vt1) * R(0.5), Re, iRe);
             T.nrt = (T.oRTold + G * D) / (R(1) + G / E);
-            if (wr && T.own) wout<R, FL>(a.oRT, a.tRT, b.o32(f, T.j, T.k), a.pp_out) = T.nrt;
+            if (wr && T.own) st_out<R, FL>(a.oRT, a.tRT, b.o32(f, T.j, T.k), a.pp_out, T.nrt);
         }
         if (T.j >= 1 && T.j <= nt - 2 && T.k >= 1 && T.k <= np - 1) {
             const R G = ((T.pG_rp + gs * L.SM[T.ty][T.tx - 1]) + gs * T.sm1) * R(0.25);
@@ -595,7 +605,7 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R D = ((T.vp1 - T.vp0) * a._dr + (tload(t.t2, (f * (nt + 1) + T.j) * NT2 + T2_e) * (T.vr1 - L.VR[T.ty][T.tx - 1])) * a._dph) -
                              div_rcp((T.vp0 + T.vp1) * R(0.5), Re, iRe);
             T.nrp = (T.oRPold + G * D) / (R(1) + G / E);
-            if (wr && T.own) wout<R, FL>(a.oRP, a.tRP, b.o32(f, T.j, T.k), a.pp_out) = T.nrp;
+            if (wr && T.own) st_out<R, FL>(a.oRP, a.tRP, b.o32(f, T.j, T.k), a.pp_out, T.nrp);
         }
     }
     L.RT[T.ty][T.tx] = T.nrt;
@@ -626,7 +636,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R dt = (L.DT[T.ty - 1][T.tx] + T.nDT) * R(0.5);
             const R v = T.vt0 + dV * dt;
             const uint32_t o = b.o32(s, T.j, T.k);
-            wout<R, FL>(a.oVT, a.VT, o, a.pp_out) = v;
+            st_out<R, FL>(a.oVT, a.VT, o, a.pp_out, v);
             if (FL & F_CHECK) {
                 const R d = v - T.vt0;
                 if (!(FL & F_CORR)) fat(a.VTe, o) = d;
@@ -651,7 +661,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             const R dt = (L.DT[T.ty][T.tx - 1] + T.nDT) * R(0.5);
             const R v = T.vp0 + dV * dt;
             const uint32_t o = b.o32(s, T.j, T.k);
-            wout<R, FL>(a.oVP, a.VP, o, a.pp_out) = v;
+            st_out<R, FL>(a.oVP, a.VP, o, a.pp_out, v);
             if (FL & F_CHECK) {
                 const R d = v - T.vp0;
                 if (!(FL & F_CORR)) fat(a.VPe, o) = d;
@@ -672,7 +682,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             R pl;
             const R e = power_law_pt(tii, T.oini, T.oPL, &pl, a, FL);
             const uint32_t o = b.o32(s, T.j, T.k);
-            wout<R, FL>(a.oEta, a.eta, o, a.ppe_out) = e;
+            st_out<R, FL>(a.oEta, a.eta, o, a.ppe_out, e);
             if (FL & F_RELAX) fat(a.oPL, o) = pl;
             if (FL & F_LAST) fat(a.TII, o) = tii;
             if (T.k == 1) fat(a.TBP, tbp_off(b, 0, s, T.j)) = tii;
