@@ -314,7 +314,7 @@ SSPT_DH inline void aphaseA(AThread &T, ALds &L, const AArgs &a, const AStepA &S
     if (is_cell(b, q, j, k)) {
         double plo;
         const double lt = adj_power_law_v(a, le, tii, ini, ePL, lPL, &plo);
-        if (a.do_relax && wr && T.own) fat(a.lo[A_PL], o) = plo;
+        if (a.do_relax && wr && T.own) fused::st_stream(fat(a.lo[A_PL], o), plo);
         if (inner_c(b, q, j, k)) {
             // Q = lam_tII / tII (compute_TII!'s transpose; 0 where tII = 0)
             const double s = lt + (next_to_boundary(b, q, j, k) ? lb : 0.0);
@@ -396,10 +396,10 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
         const double id = 1.0 / den;
         const double vRR = nRR * id, vTT = nTT * id, vPP = nPP * id;
         if (wr && T.own) {
-            fat(a.lo[A_P], o) = vP;
-            fat(a.lo[A_RR], o) = vRR;
-            fat(a.lo[A_TT], o) = vTT;
-            fat(a.lo[A_PP], o) = vPP;
+            fused::st_stream(fat(a.lo[A_P], o), vP);
+            fused::st_stream(fat(a.lo[A_RR], o), vRR);
+            fused::st_stream(fat(a.lo[A_TT], o), vTT);
+            fused::st_stream(fat(a.lo[A_PP], o), vPP);
         }
         // compute_P!'s centre stencil weights for vel
         const double GS2 = 2.0 * T.GS;
@@ -425,7 +425,7 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
         const double g4 = (((L.GSM[pv][ty - 1][tx] + L.GSM[pv][ty][tx]) + L.GSM[c][ty - 1][tx]) + L.GSM[c][ty][tx]) * 0.25;
         const double e4 = (((L.ETA[pv][ty - 1][tx] + L.ETA[pv][ty][tx]) + L.ETA[c][ty - 1][tx]) + L.ETA[c][ty][tx]) * 0.25;
         const double v = ((T.lRT + s) + tq) * (1.0 / (1.0 + g4 / e4));
-        if (wr && T.own) fat(a.lo[A_RT], o) = v;
+        if (wr && T.own) fused::st_stream(fat(a.lo[A_RT], o), v);
         T.gRT = g4 * v;
         T.cRT = (0.25 * T.tRT * (g4 / (e4 * e4))) * v;
     }
@@ -436,7 +436,7 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
         const double g4 = (((L.GSM[pv][ty][tx - 1] + L.GSM[pv][ty][tx]) + L.GSM[c][ty][tx - 1]) + L.GSM[c][ty][tx]) * 0.25;
         const double e4 = (((L.ETA[pv][ty][tx - 1] + L.ETA[pv][ty][tx]) + L.ETA[c][ty][tx - 1]) + L.ETA[c][ty][tx]) * 0.25;
         const double v = ((T.lRP + s) + tq) * (1.0 / (1.0 + g4 / e4));
-        if (wr && T.own) fat(a.lo[A_RP], o) = v;
+        if (wr && T.own) fused::st_stream(fat(a.lo[A_RP], o), v);
         T.gRP = g4 * v;
         T.cRP = (0.25 * T.tRP * (g4 / (e4 * e4))) * v;
     }
@@ -449,7 +449,7 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
         const double g4 = (((L.GSM[c][ty - 1][tx - 1] + L.GSM[c][ty - 1][tx]) + L.GSM[c][ty][tx - 1]) + L.GSM[c][ty][tx]) * 0.25;
         const double e4 = (((L.ETA[c][ty - 1][tx - 1] + L.ETA[c][ty - 1][tx]) + L.ETA[c][ty][tx - 1]) + L.ETA[c][ty][tx]) * 0.25;
         const double v = ((T.lTP + s) + tq) * (1.0 / (1.0 + g4 / e4));
-        if (wr && T.own) fat(a.lo[A_TP], o) = v;
+        if (wr && T.own) fused::st_stream(fat(a.lo[A_TP], o), v);
         T.gTP = g4 * v;
         T.cTP = (0.25 * T.tTP * (g4 / (e4 * e4))) * v;
     }
@@ -482,9 +482,9 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
     if (wr_m && T.own && q >= 1) {
         const uint32_t om = b.o32(q - 1, T.jc, T.kc);
         const double mR = -a._dr - 0.5 * iRe;
-        if (j >= 0 && j <= nt && ink) fat(a.lo[A_VT], om) = L.ACC[ACC_VT][ty][tx] + T.gRT * mR;
-        if (inj && k >= 0 && k <= np) fat(a.lo[A_VP], om) = L.ACC[ACC_VP][ty][tx] + T.gRP * mR;
-        if (inj && ink) fat(a.lo[A_E], om) = ((L.ACC[ACC_E][ty][tx] + T.cRT) + cRTj) + (T.cRP + cRPk);
+        if (j >= 0 && j <= nt && ink) fused::st_stream(fat(a.lo[A_VT], om), L.ACC[ACC_VT][ty][tx] + T.gRT * mR);
+        if (inj && k >= 0 && k <= np) fused::st_stream(fat(a.lo[A_VP], om), L.ACC[ACC_VP][ty][tx] + T.gRP * mR);
+        if (inj && ink) fused::st_stream(fat(a.lo[A_E], om), ((L.ACC[ACC_E][ty][tx] + T.cRT) + cRTj) + (T.cRP + cRPk));
     }
     // ---- VR(q): complete
     if (q <= nr && inj && ink) {
@@ -497,7 +497,7 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
         v += T.gRP * re;
         v -= gRPk * re;
         if (a.bJ && in_objective(a, q, j, k)) v += fat(a.bJ, b.o32(qt, T.jc, T.kc));
-        if (wr && T.own) fat(a.lo[A_VR], b.o32(qt, T.jc, T.kc)) = v;
+        if (wr && T.own) fused::st_stream(fat(a.lo[A_VR], b.o32(qt, T.jc, T.kc)), v);
     }
     L.ACC[ACC_VRN][ty][tx] = is_cell(b, q, j, k) ? T.cw_dR * a._dr + 0.5 * T.cw_aR : 0.0;
     // ---- Vth(q), Vph(q), eta(q) without the edges of box index q+1

@@ -193,15 +193,23 @@ template <typename R, int FL> SSPT_DH inline R &wout(R *out, const R *in, uint32
     else
         return fat(out, o);
 }
-// Stores a new state value (SSPT_NT_STORE: with the non-temporal hint, for
-// measurements of the cache policy of the output stream).
+// Store of an output that only the next launch reads.  SSPT_NT_STORE adds the
+// non-temporal hint (global_store ... nt), a build option for measurements:
+// at 511^3 it measured no gain on the FP64 iteration and made the FP32 sweep
+// 10 % and the adjoint step 1.6 % slower (both builds in one process,
+// alternating repetitions).
+template <typename T> SSPT_DH inline void st_stream(T &dst, T v)
+{
+#if defined(__HIP_DEVICE_COMPILE__) && defined(SSPT_NT_STORE)
+    __builtin_nontemporal_store(v, &dst);
+#else
+    dst = v;
+#endif
+}
+// Stores a new state value.
 template <typename R, int FL> SSPT_DH inline void st_out(R *out, const R *in, uint32_t o, uint32_t pp, R v)
 {
-#if defined(SSPT_NT_STORE) && defined(__HIP_DEVICE_COMPILE__)
-    __builtin_nontemporal_store(v, &wout<R, FL>(out, in, o, pp));
-#else
-    wout<R, FL>(out, in, o, pp) = v;
-#endif
+    st_stream(wout<R, FL>(out, in, o, pp), v);
 }
 
 template <typename R> struct Lds {
