@@ -46,7 +46,12 @@ def _solver_with(setup: Setup, rho: np.ndarray, log10_ini: np.ndarray) -> Solver
 
 class AscentVelocity(torch.autograd.Function):
     """J = AscentVelocity.apply(rho, log10_eta_ini, setup, options); rho and
-    log10_eta_ini are float64 tensors of the centre shape (nr, nth, nph)."""
+    log10_eta_ini are float64 CPU tensors of the centre shape (nr, nth, nph).
+
+    The solve runs in libsspt.so, which uses the system HIP runtime; torch
+    carries its own, and in one process only the runtime that opens the GPU
+    first sees it, so the tensors here stay on the host (the fields are
+    uploaded through the C ABI)."""
 
     @staticmethod
     def forward(ctx, rho, log10_eta_ini, setup: Setup, opts: SolveOptions):
