@@ -220,5 +220,21 @@ class Oracle:
         return it, arrs, g_rho, g_ini
 
 
+def initialize_inclusion(A, R, th, ph, r, radius, val):
+    """orc_initialize_inclusion on the Fortran-ordered array A (in place)."""
+    assert A.flags.f_contiguous and A.dtype == np.float64
+    R, th, ph = (np.ascontiguousarray(x, dtype=np.float64) for x in (R, th, ph))
+    lib().orc_initialize_inclusion(_ptr(A), _ptr(R), _ptr(th), _ptr(ph), A.shape[0], A.shape[1], A.shape[2],
+                                   float(r), float(radius), float(val))
+
+
+def initialize_velocity(Vph, th, fact, r):
+    """orc_initialize_velocity on the Fortran-ordered Vph (nr, nth, nph+1), in place."""
+    assert Vph.flags.f_contiguous and Vph.dtype == np.float64
+    th = np.ascontiguousarray(th, dtype=np.float64)
+    lib().orc_initialize_velocity(_ptr(Vph), _ptr(th), Vph.shape[0], Vph.shape[1], Vph.shape[2] - 1, float(fact),
+                                  float(r))
+
+
 def num_threads() -> int:
     return lib().orc_num_threads()

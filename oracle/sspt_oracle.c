@@ -395,7 +395,12 @@ void orc_bc_R(orc_state *s)
  *                   Vph = (th-pi/2)*r*sh, last Vph column uses th[..,nph])
  *  ORC_BC_FREE_SLIP SphericalStokes_plateau.jl:74-82
  *  ORC_BC_PURE_SHEAR extension: normal velocity prescribed at the boundary
- *                   centre, Vth_b = -((th_b-pi/2)*r)*sh, tangential copied. */
+ *                   centre, Vth_b = -(((th_b-pi/2)*r)*sh)/sin(th_b),
+ *                   tangential copied.  With Vph_b = (ph_b*r)*sh on the phi
+ *                   faces (orc_bc_ph) these are the normal components of
+ *                   V = (0, -sh*r*(th-pi/2)/sin(th), sh*r*ph), a
+ *                   divergence-free pure shear of rate sh in the th-phi
+ *                   plane, so the prescribed boundary fluxes cancel. */
 void orc_bc_th(orc_state *s, const orc_params *p)
 {
     const int nr = s->nr, nt = s->nth, np = s->nph;
@@ -418,8 +423,8 @@ void orc_bc_th(orc_state *s, const orc_params *p)
     for (int iz = 1; iz <= np; ++iz)
         for (int ix = 1; ix <= nr; ++ix) {
             if (mode == ORC_BC_PURE_SHEAR) {
-                const double v0 = -(((T1[1] - PI / 2.0) * r) * sh);
-                const double v1 = -(((T1[nt] - PI / 2.0) * r) * sh);
+                const double v0 = -((((T1[1] - PI / 2.0) * r) * sh) / sin(T1[1]));
+                const double v1 = -((((T1[nt] - PI / 2.0) * r) * sh) / sin(T1[nt]));
                 VT[IDX(nr, nt + 1, ix, 1, iz)] = 2.0 * v0 - VT[IDX(nr, nt + 1, ix, 2, iz)];
                 VT[IDX(nr, nt + 1, ix, nt + 1, iz)] = 2.0 * v1 - VT[IDX(nr, nt + 1, ix, nt, iz)];
             } else {

@@ -7,6 +7,12 @@ inclusion of the converged PT solve.  forward runs the fused PT loop with the
 reference exit test (ss_pt_solve) and keeps the converged state on the GPU;
 backward runs the adjoint solve on that state (ss_adjoint_solve): no record
 of the forward iterations is kept.
+
+The adjoint is exact only at a fixed point of the PT iteration, so forward
+raises NotConvergedError when the PT loop ends without meeting its exit
+test, and backward raises it when the adjoint iteration does not reach its
+tolerance (unless SolveOptions.strict is False, in which case the statuses
+are only reported in SolveOptions.info).
 """
 from __future__ import annotations
 
@@ -30,7 +36,12 @@ class SolveOptions:
     adj_maxiter: int = 4_000_000
     adj_ncheck: int = 200
     adj_tol: float = 1e-13
+    strict: bool = True          # raise NotConvergedError on an unconverged forward or adjoint solve
     info: dict = field(default_factory=dict)
+
+
+class NotConvergedError(RuntimeError):
+    """A solve that the gradient relies on did not converge."""
 
 
 def _solver_with(setup: Setup, rho: np.ndarray, log10_ini: np.ndarray) -> Solver:
@@ -57,10 +68,15 @@ class AscentVelocity(torch.autograd.Function):
     def forward(ctx, rho, log10_eta_ini, setup: Setup, opts: SolveOptions):
         s = _solver_with(setup, rho.detach().cpu().numpy().astype(np.float64),
                          log10_eta_ini.detach().cpu().numpy().astype(np.float64))
-        res = s.pt_solve(setup.params, opts.maxiter, opts.nout, opts.eps)
+        res = s.pt_solve(setup.params, opts.maxiter, opts.nout, opts.eps, raise_on=())
+        opts.info["forward"] = res
+        if res["status"] != "SS_OK" or res["iter_end"] == 0:
+            if opts.strict or res["status"] == "SS_ERR_NONFINITE":
+                s.close()
+                raise NotConvergedError(f"forward PT solve: {res['status']} after {res['ittot']} iterations "
+                                        f"(err {res['err']:.3e}, eps {opts.eps:.1e})")
         s.adjoint_prepare(setup.params)
         J = s.adjoint_objective(opts.r, opts.radius)
-        opts.info["forward"] = res
         if not any(ctx.needs_input_grad[:2]):      # no backward will come: release the device state now
             s.close()
             s = None
@@ -77,6 +93,9 @@ class AscentVelocity(torch.autograd.Function):
             s.close()
             ctx.solver = None
         o.info["adjoint"] = {k: v for k, v in r.items() if not k.startswith("grad")}
+        if o.strict and not r["converged"]:
+            raise NotConvergedError(f"adjoint solve: not converged after {r['iters']} iterations "
+                                    f"(err {r['err']:.3e}, tol {o.adj_tol:.1e})")
         g = float(gJ)
         to = lambda a: torch.as_tensor(np.ascontiguousarray(a) * g, dtype=ctx.dtype, device=ctx.device)
         return to(r["grad_rho"]), to(r["grad_log10_eta_ini"]), None, None

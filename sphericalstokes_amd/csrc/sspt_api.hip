@@ -283,6 +283,8 @@ int ss_set_grid(ss_ctx *c, const double *R1, const double *th1, const double *ph
     const double *thp = h.data() + (h.size() - (c->box.nt + 1) - (c->box.np + 1));
     h32.insert(h32.end(), thp, thp + (c->box.nt + 1) + (c->box.np + 1));
     HIP_TRY(c, hipMemcpyAsync(c->ftab32_mem, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    c->th_lo = th1[0];
+    c->th_hi = th1[c->box.nt - 1];
     int rc = sync(c, "ss_set_grid");
     if (rc == SS_OK) c->grid_set = true;
     return rc;

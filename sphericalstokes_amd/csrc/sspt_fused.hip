@@ -203,6 +203,8 @@ static Args<double> make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out
     a.bc_mode = p.bc_mode;
     a.r = p.r;
     a.sh = p.sh;
+    a.vth0 = pure_shear_vth(c->th_lo, p.r, p.sh);
+    a.vth1 = pure_shear_vth(c->th_hi, p.r, p.sh);
     a.rchunk = c->box.nr;
     return a;
 }

@@ -141,6 +141,9 @@ template <typename R> struct Args {
     int flags;
     int bc_mode;
     R r, sh;
+    // pure shear: prescribed Vth at the th = th[0] and th = th[nt-1] boundaries
+    // (pure_shear_vth, evaluated on the host)
+    R vth0, vth1;
     int rchunk; // r-steps per block along grid z
     // byte offsets of the in and out halves of the state fields (viscosity
     // apart: it is frozen in a correction sweep); 0 unless F_PAIRED, where
@@ -220,6 +223,16 @@ template <typename R> struct Lds {
     R RT[TJB<R>][TK], RP[TJB<R>][TK], TP[TJB<R>][TK];
 };
 
+// Normal velocity that the pure-shear BC prescribes on a th boundary at th_b:
+// the th component of V = (0, -sh*r*(th-pi/2)/sin(th), sh*r*ph), a
+// divergence-free pure shear of rate sh in the th-phi plane whose phi
+// component is the phi faces' prescribed Vph_b = (ph_b*r)*sh.  Host code
+// only: sin comes from the host math library, as in the CPU restatement.
+inline double pure_shear_vth(double th_b, double r, double sh)
+{
+    return -((((th_b - PI_ / 2.0) * r) * sh) / std::sin(th_b));
+}
+
 // Julia Base.max/min for Float64
 template <typename R> SSPT_DH inline R jmax(R x, R y)
 {
@@ -252,17 +265,30 @@ template <typename R> SSPT_DH inline R div_rcp(R x, R d, R inv_d)
     return std::fma(r, inv_d, q);
 }
 
-// x^ex of the power law.  ex = 0 (n_exp = 1) is 1 exactly, as pow returns;
-// small integer exponents (n_exp = 2..9 -> ex = -1..-8) use repeated
-// products and one reciprocal (within 2 ulp of pow); other exponents use pow.
+// x^ex of the power law.  ex = 0 (n_exp = 1) is 1 exactly, as pow returns.
+// Small integer exponents (n_exp = 2..9 -> ex = -1..-8): x^m is formed as
+// an unevaluated sum hi + lo (each product's rounding error recovered with
+// an FMA), then 1/(hi + lo) as q = RN(1/hi) plus one Newton correction
+// q + q*e, e = 1 - q*(hi + lo), whose relative error before the last
+// rounding is about 2^-100: the result is the correctly rounded x^-m except
+// within ~2^-47 of a rounding tie.  A zero, infinite or NaN q (x = 0,
+// overflow, NaN) is returned as is, as pow does (pow(0, -m) = +inf).  Other
+// exponents use pow.
 template <typename R> SSPT_DH inline R pow_ex(R x, R ex)
 {
     if (ex == R(0)) return R(1);
     if (ex < R(0) && ex >= R(-8) && ex == (R)(int)ex) {
         const int m = -(int)ex;
-        R p = x;
-        for (int i = 1; i < m; ++i) p = p * x;
-        return R(1) / p;
+        R hi = x, lo = R(0);
+        for (int i = 1; i < m; ++i) {
+            const R h = hi * x;
+            lo = std::fma(hi, x, -h) + lo * x;
+            hi = h;
+        }
+        const R q = R(1) / hi;
+        if (!(std::fabs(q) < R(INFINITY)) || q == R(0)) return q;
+        const R e = std::fma(-q, hi, R(1)) - q * lo;
+        return std::fma(q, e, q);
     }
     return std::pow(x, ex);
 }
@@ -743,11 +769,7 @@ template <typename R> SSPT_DH inline R bc_apply(const Args<R> &A, int comp, int 
     if (dir == 0) return comp == 0 ? -v : v;
     if (dir == 1) {
         if (comp == 1) {
-            if (mode == 2) {
-                const R th = side ? A.t.th[b.nt - 1] : A.t.th[0];
-                const R vb = -(((th - R(PI_ / 2.0)) * A.r) * A.sh);
-                return R(2) * vb - v;
-            }
+            if (mode == 2) return R(2) * (side ? A.vth1 : A.vth0) - v;
             return -v;
         }
         if (mode == 0) {

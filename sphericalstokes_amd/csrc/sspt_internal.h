@@ -135,6 +135,7 @@ struct ss_ctx {
     double adj_obj_r = 0.0, adj_obj_rad = 0.0; // inclusion of the objective (ss_adjoint_objective)
     bool adj_fused = true;          // r-marching adjoint kernel, else the per-family kernels (set by ss_adjoint_prepare)
     bool grid_set = false;
+    double th_lo = 0.0, th_hi = 0.0; // th[0], th[nth-1] (host copies for the pure-shear BC)
     double *staging = nullptr;      // upload/download staging (largest field)
     size_t staging_elems = 0;
     unsigned long long *red = nullptr; // reduction slots (device)

@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(256) k_bc_R(Box b, double *__restrict__ VR, do
 // ---------------------------------------------------------------- BC_th!
 __global__ void __launch_bounds__(256) k_bc_th(Box b, double *__restrict__ VR, double *__restrict__ VT,
                                                double *__restrict__ VP, const double *__restrict__ th, int mode,
-                                               double r, double sh)
+                                               double r, double sh, double v0, double v1)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x; // 0..np
     const int i = blockIdx.y * blockDim.y + threadIdx.y; // 0..nr
@@ -288,9 +288,7 @@ __global__ void __launch_bounds__(256) k_bc_th(Box b, double *__restrict__ VR, d
         }
     }
     if (i < nr && k < np) {
-        if (mode == SS_BC_PURE_SHEAR) {
-            const double v0 = -(((th[0] - PI / 2.0) * r) * sh);
-            const double v1 = -(((th[nt - 1] - PI / 2.0) * r) * sh);
+        if (mode == SS_BC_PURE_SHEAR) { // v0, v1: prescribed Vth at th[0], th[nt-1]
             VT[b.off(i, 0, k)] = 2.0 * v0 - VT[b.off(i, 1, k)];
             VT[b.off(i, nt, k)] = 2.0 * v1 - VT[b.off(i, nt - 1, k)];
         } else {
@@ -604,7 +602,8 @@ void launch_bc_th(ss_ctx *c, const FieldSet &s, int mode, double r, double sh)
 {
     const Box &b = c->box;
     hipLaunchKernelGGL(k_bc_th, dim3((b.np + 1 + 63) / 64, (b.nr + 1 + 3) / 4), B2, 0, c->stream, b, s.f[SS_VR],
-                       s.f[SS_VTH], s.f[SS_VPH], c->tab.th, mode, r, sh);
+                       s.f[SS_VTH], s.f[SS_VPH], c->tab.th, mode, r, sh, fused::pure_shear_vth(c->th_lo, r, sh),
+                       fused::pure_shear_vth(c->th_hi, r, sh));
 }
 
 void launch_bc_ph(ss_ctx *c, const FieldSet &s, int mode, double r, double sh)

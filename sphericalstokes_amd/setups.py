@@ -83,6 +83,28 @@ def _velocity_shear(Vph, th, r, fact):
     Vph[...] = (((th - math.pi / 2.0) * r) * fact)[None, :, None]
 
 
+def _faces(x):
+    """Face positions of a uniform centre grid x: x[0]-dx/2, midpoints, x[-1]+dx/2."""
+    dx = x[1] - x[0]
+    return np.concatenate([[x[0] - 0.5 * dx], 0.5 * (x[:-1] + x[1:]), [x[-1] + 0.5 * dx]])
+
+
+def pure_shear_velocity(th, ph, r, sh):
+    """The far-field pure shear V = (0, -sh*r*(th-pi/2)/sin(th), sh*r*ph) at
+    the staggered Vth and Vph positions.  It is divergence-free, and its
+    normal components on the th and phi boundaries are what the pure-shear
+    BC prescribes there (a build extension; the reference has simple shear
+    only).  Returns (Vth[nth+1], Vph[nph+1]) as 1D profiles."""
+    thf, phf = _faces(th), _faces(ph)
+    return -(((thf - math.pi / 2.0) * r) * sh) / np.sin(thf), (phf * r) * sh
+
+
+def _velocity_pure_shear(Vth, Vph, th, ph, r, sh):
+    vt, vp = pure_shear_velocity(th, ph, r, sh)
+    Vth[...] = vt[None, :, None]
+    Vph[...] = vp[None, None, :]
+
+
 def diapir(nr, nth, nph, *, eta0=1.0, vr=1e2, rho0=0.0, drho=-10.0, g=1.0, n_exp=5.0, tau_C=1.0,
            r=1000.0, lr=6.0, lth=None, lph=None, radius=1.0, sh=1.0, dmp=4.5, bc_mode=BC_SHEAR,
            relax=1.0, eta_min=0.0, eta_max=math.inf, init_shear=None) -> Setup:
@@ -99,8 +121,13 @@ def diapir(nr, nth, nph, *, eta0=1.0, vr=1e2, rho0=0.0, drho=-10.0, g=1.0, n_exp
     f["eta"][...] = eta0
     _inclusion(f["rho"], R, th, ph, r, radius, rho0 + drho)
     _inclusion(f["eta"], R, th, ph, r, radius, eta0 / vr)
-    if init_shear if init_shear is not None else bc_mode == BC_SHEAR:
-        _velocity_shear(f["Vph"], th, r, sh)
+    # far-field flow as the initial guess (initialize_velocity!, :403), for
+    # both kinematic BCs; free slip starts at rest
+    if init_shear if init_shear is not None else bc_mode in (BC_SHEAR, BC_PURE_SHEAR):
+        if bc_mode == BC_PURE_SHEAR:
+            _velocity_pure_shear(f["Vth"], f["Vph"], th, ph, r, sh)
+        else:
+            _velocity_shear(f["Vph"], th, r, sh)
     f["eta_ini"][...] = f["eta"]
     f["eta_PL"][...] = f["eta"]
     p = Params(_dr=(nr - 1) / lr, _dth=(nth - 1) / lth, _dph=(nph - 1) / lph, dmp=dmp, vpdt=dr * CFL,
@@ -208,7 +235,12 @@ def plateau_sector(n, *, nth=None, nph=None, n_crust=3.0, eta_mantle=1e-2, rho_c
 
 def random_state(s: Setup, seed: int = 0, scale: float = 1e-2) -> Setup:
     """Fills V, P and stresses with seeded random values (parity tests on a
-    non-trivial state without running a solve first)."""
+    non-trivial state without running a solve first).  The auxiliary fields
+    that single kernels read (etaSM for compute_P!, DT_R for compute_V!, tII
+    for power_law!) get positive values of a realistic size, so that every
+    per-kernel call changes its outputs: etaSM the 7-point maximum of eta
+    times a random factor in [1, 2), DT_R of the size of vpdt^2/etaSM, tII
+    of the size of the stresses."""
     rng = np.random.default_rng(seed)
     for n in ("VR", "Vth", "Vph", "P", "tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph"):
         a = s.fields[n]
@@ -218,4 +250,12 @@ def random_state(s: Setup, seed: int = 0, scale: float = 1e-2) -> Setup:
     s.fields["eta_ini"][...] = np.exp(rng.standard_normal(e.shape))
     s.fields["eta_PL"][...] = s.fields["eta"]
     s.fields["rho"][...] = rng.standard_normal(e.shape)
+    m = e.copy()
+    for ax in range(3):
+        for sh in (1, -1):
+            m = np.maximum(m, np.roll(e, sh, axis=ax))
+    s.fields["etaSM"][...] = m * (1.0 + rng.random(e.shape))
+    p = s.params
+    s.fields["DT_R"][...] = (p.vpdt * p.vpdt) / s.fields["etaSM"] * (0.5 + rng.random(e.shape))
+    s.fields["tII"][...] = scale * np.exp(rng.standard_normal(e.shape))
     return s

@@ -28,6 +28,8 @@ def lib():
         L.emu_fused_iterations.argtypes = [C.c_int] * 4 + [DP] * 3 + [C.POINTER(ss_params), C.POINTER(DP),
                                            C.POINTER(DP), C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_ulonglong), C.c_longlong]
+        L.emu_pow_ex.restype = None
+        L.emu_pow_ex.argtypes = [DP, C.c_int, C.c_double, DP]
         _L = L
     return _L
 
@@ -130,4 +132,13 @@ def adj_step(setup, state, lam, params, rchunk=0):
         o = 1 if n in SHEAR else 0
         s = lam[n].shape
         out[n] = np.asfortranarray(x[o:o + s[0], o:o + s[1], o:o + s[2]])
+    return out
+
+
+def pow_ex(x, ex):
+    """The fused kernels' x^ex (sspt_fused_body.h pow_ex), run on the host."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    DP = C.POINTER(C.c_double)
+    lib().emu_pow_ex(x.ctypes.data_as(DP), x.size, float(ex), out.ctypes.data_as(DP))
     return out

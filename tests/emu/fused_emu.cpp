@@ -120,6 +120,12 @@ template <typename R> void run_boundary(const Args<R> &a, unsigned long long red
 
 extern "C" {
 
+// x^ex through the kernels' pow_ex for n values (tests of its rounding)
+void emu_pow_ex(const double *x, int n, double ex, double *out)
+{
+    for (int q = 0; q < n; ++q) out[q] = pow_ex(x[q], ex);
+}
+
 // Runs `niter` fused iterations on box-layout buffers (shape (nr+1) x (nth+1)
 // x pitch, phi fastest).  cur/alt: SS_NFIELDS pointers each; non-state fields
 // must alias between the two sets.  The last iteration writes the auxiliary
@@ -175,6 +181,8 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
         a.flags = (last ? F_LAST : 0) | (check ? F_CHECK : 0) | (p->relax != 1.0 ? F_RELAX : 0) |
                   ((p->eta_min > 0.0 || p->eta_max < INFINITY) ? F_CLAMP : 0);
         a.bc_mode = p->bc_mode; a.r = p->r; a.sh = p->sh;
+        a.vth0 = pure_shear_vth(th1[0], p->r, p->sh);
+        a.vth1 = pure_shear_vth(th1[nt - 1], p->r, p->sh);
         a.rchunk = rchunk > 0 ? rchunk : nr;
         Args<double> am = a;
         if (pair_bytes > 0 && !pair_args(am, pair_bytes)) return -1;

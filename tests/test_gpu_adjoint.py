@@ -171,3 +171,52 @@ def test_autograd_function_matches_central_differences():
         ad = float(np.sum(grad * dm))
         say(name, "fd", fd, "autograd", ad)
         assert abs(fd - ad) <= 1e-5 * abs(fd), (name, fd, ad)
+
+
+def test_adjoint_gradient_config2_63_sixteen_directions(Solver):
+    """The adjoint configuration's check at 63^3: the power-law diapir under
+    far-field pure shear (n_exp = 3, cutoffs 1e-4..1e4, relaxation 0.5),
+    J = mean VR in the inclusion; the gradient with respect to rho and
+    log10(eta_ini) against central differences (h = 1e-6) along 16 Gaussian
+    directions per parameter from seed 1, within 1e-5.  Each perturbed solve
+    restarts from the converged state and runs a fixed 40 000 iterations (the
+    part of J's error that does not cancel between the two signs is about
+    h * rho^K, rho the PT contraction); the 64 solves run four at a time on
+    separate contexts (a 63^3 launch fills few of the CUs)."""
+    from concurrent.futures import ThreadPoolExecutor
+    st = setups.diapir_sector(63, n_exp=3.0, bc_mode=BC_PURE_SHEAR, sh=1.0, relax=0.5, eta_min=1e-4, eta_max=1e4)
+    p = st.params
+    with _load(Solver, st, st.fields) as s:
+        f = s.pt_solve(p, 2_000_000, 500, 1e-13)
+        say("forward", f)
+        assert f["status"] == "SS_OK"
+        base = s.download_all()
+        r = s.adjoint_solve(p, R0, RAD, maxiter=2_000_000, ncheck=500, tol=1e-13)
+    say("adjoint", {k: v for k, v in r.items() if not k.startswith("grad")})
+    assert r["converged"]
+    rng = np.random.default_rng(1)
+    dirs = [(what, rng.standard_normal(base["rho"].shape)) for what in ("rho", "log10_eta_ini") for _ in range(16)]
+    h, iters = 1e-6, 40_000
+
+    def J_of(what, dm, sgn):
+        g = {k: v.copy(order="F") for k, v in base.items()}
+        if what == "rho":
+            g["rho"] = np.asfortranarray(g["rho"] + sgn * h * dm)
+        else:
+            g["eta_ini"] = np.asfortranarray(g["eta_ini"] * 10.0 ** (sgn * h * dm))
+        with _load(Solver, st, g) as s2:
+            s2.pt_iterate(p, iters)
+            s2.adjoint_prepare(p)
+            return s2.adjoint_objective(R0, RAD)
+
+    with ThreadPoolExecutor(4) as ex:
+        futs = [(what, dm, ex.submit(J_of, what, dm, +1), ex.submit(J_of, what, dm, -1)) for what, dm in dirs]
+        worst = 0.0
+        for q, (what, dm, fp, fm) in enumerate(futs):
+            fd = (fp.result() - fm.result()) / (2 * h)
+            ad = float(np.sum((r["grad_rho"] if what == "rho" else r["grad_log10_eta_ini"]) * dm))
+            rel = abs(fd - ad) / abs(fd)
+            worst = max(worst, rel)
+            say(f"  {what:14s} direction {q % 16:2d}: fd {fd:+.10e} adjoint {ad:+.10e} rel {rel:.2e}")
+            assert rel <= 1e-5, (what, q, fd, ad)
+    say("worst relative difference over 32 directions", worst)

@@ -1,0 +1,165 @@
+"""GPU, the benchmark configurations at their own sizes against the CPU oracle.
+
+* Linear rising-sphere diapir, 63^3 (free slip, n_exp = 1): 2000 fused
+  iterations bit for bit; the reference loop with its exit test (err every
+  100 iterations, eps = 1e-8, at most 1e5 iterations) against the oracle's
+  converged run recorded in tests/golden/config1_63_oracle.json (exit-test
+  record, ascent velocity and a SHA-256 of every field); the solve to an
+  FP64 residual of 1e-8 with physical checks of the result.
+* Power-law diapir under far-field pure shear, 255^3 (n_exp = 3, viscosity
+  cutoffs 1e-4..1e4, relaxation 0.5): every field after 2000 fixed fused
+  iterations within rel-L2 1e-10 of the oracle, with the power law active
+  (finite eta_PL, eta != 2 eta_ini).
+* Plateau, 511^3 (power-law crust, free slip): 3 fused iterations against
+  the oracle over the full arrays.
+"""
+import hashlib
+import json
+import math
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+from oracle.binding import Oracle
+from sphericalstokes_amd import setups
+from sphericalstokes_amd.fields import BC_PURE_SHEAR, FIELDS
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10   # rel-L2 per field, FP64 drop-in tolerance
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _solver(st):
+    from sphericalstokes_amd.solver import Solver
+    s = Solver(st.nr, st.nth, st.nph)
+    s.load_setup(st)
+    return s
+
+
+def _ascent(s, st):
+    """Ascent velocity: the volume-weighted mean VR over the r-faces inside
+    the sphere of radius 0.1 at (1.5, pi/2, 0) (the adjoint's objective)."""
+    s.adjoint_prepare(st.params)
+    return s.adjoint_objective(1.5, 0.1 ** 2)
+
+
+def _compare_all(s, o, tol=TOL_FP64, exact=False):
+    worst = 0.0
+    for name in FIELDS:
+        g, r = s.download(name), o.f[name]
+        fa, fb = np.isfinite(g), np.isfinite(r)
+        assert np.array_equal(fa, fb) and np.array_equal(g[~fa], r[~fb], equal_nan=True), name
+        e = rel_l2(g[fa], r[fb])
+        nd = int(np.count_nonzero(g[fa] != r[fb]))
+        say(f"  {name:9s} rel-L2 {e:.3e}  differing entries {nd}")
+        worst = max(worst, e)
+        if exact:
+            assert nd == 0, name
+        assert e <= tol, (name, e)
+    return worst
+
+
+# ---------------------------------------------------------------- 63^3 linear diapir
+def test_config1_63_fixed_iterations_bitwise():
+    st = setups.diapir_sector(63, n_exp=1.0)
+    say("63^3 linear diapir: 2000 fused iterations vs the oracle")
+    o = Oracle(st)
+    s = _solver(st)
+    t0 = time.time()
+    for done in range(500, 2001, 500):
+        o.step(500)
+        s.pt_iterate(st.params, 500)
+        say(f"  {done} iterations, {time.time() - t0:.0f} s")
+    _compare_all(s, o, tol=0.0, exact=True)
+    s.close()
+
+
+def test_config1_63_reference_loop_matches_oracle_run():
+    fx = json.loads((GOLDEN / "config1_63_oracle.json").read_text())
+    st = setups.diapir_sector(63, n_exp=1.0)
+    s = _solver(st)
+    r = s.pt_solve(st.params, fx["maxiter"], fx["nout"], fx["eps"])
+    say("GPU loop", r)
+    say("oracle  ", {k: fx[k] for k in ("ittot", "iter_end", "err", "J")})
+    assert r["ittot"] == fx["ittot"] and r["iter_end"] == fx["iter_end"] and r["err"] == fx["err"]
+    assert r["status"] == "SS_OK"
+    for name in FIELDS:
+        a = np.asfortranarray(s.download(name))
+        h = hashlib.sha256(a.tobytes(order="F")).hexdigest()
+        assert h == fx["fields"][name]["sha256"], (name, float(np.linalg.norm(a.ravel())), fx["fields"][name]["l2"])
+    J = _ascent(s, st)
+    say(f"ascent velocity GPU {J!r}, oracle {fx['J']!r}")
+    assert abs(J - fx["J"]) <= 1e-12 * abs(fx["J"])      # same fields, another summation order
+    s.close()
+
+
+def test_config1_63_residual_solve():
+    """Solve to an FP64 residual of 1e-8 (max|div sigma - rho g| / max|rho g|
+    and max|divV| lr / max|V|).  The buoyant sphere (drho = -1, radius a = 0.1,
+    viscosity ratio 1e-3) rises: the mean VR inside it is positive and of the
+    order of the Hadamard-Rybczynski velocity U = 2/3 drho g a^2 / eta *
+    (eta + eta_i) / (2 eta + 3 eta_i), reduced by the walls 4 radii away."""
+    st = setups.diapir_sector(63, n_exp=1.0)
+    s = _solver(st)
+    r = s.pt_solve_residual(st.params, 600_000, 500, 1e-8)
+    say("residual solve", r)
+    assert r["converged"] and r["r"] <= 1e-8
+    res = s.residual(st.params)
+    assert res[0] <= 1e-8 and res[1] <= 1e-8 and res[2] <= 1e-8
+    J = _ascent(s, st)
+    a, eta, eta_i = 0.1, 1.0, 1e-3
+    U = 2.0 / 3.0 * a * a / eta * (eta + eta_i) / (2 * eta + 3 * eta_i)
+    say(f"ascent velocity {J:.5e}, Hadamard-Rybczynski {U:.5e}, ratio {J / U:.3f}")
+    assert math.isfinite(J) and 0.2 * U < J < 1.2 * U
+    # away from the inclusion the flow returns downwards: VR changes sign
+    VR = s.download("VR")
+    assert VR.max() > 0 and VR.min() < 0
+    s.close()
+
+
+# ---------------------------------------------------------------- 255^3 power-law pure shear
+def test_config2_255_powerlaw_pure_shear_2000_iterations():
+    n, iters, chunk = 255, 2000, 100
+    st = setups.diapir_sector(n, n_exp=3.0, bc_mode=BC_PURE_SHEAR, sh=1.0, relax=0.5, eta_min=1e-4, eta_max=1e4)
+    say("255^3 power-law diapir, pure shear: 2000 fused iterations vs the oracle")
+    o = Oracle(st)
+    s = _solver(st)
+    t0 = time.time()
+    for done in range(chunk, iters + 1, chunk):
+        s.pt_iterate(st.params, chunk)
+        o.step(chunk)
+        say(f"  {done} iterations, {time.time() - t0:.0f} s")
+    eta, ini, pl = o.f["eta"], o.f["eta_ini"], o.f["eta_PL"]
+    inner = (slice(1, -1),) * 3
+    fin = float(np.isfinite(pl).mean())
+    act = float((eta[inner] != 2.0 * ini[inner]).mean())
+    say(f"power law: eta_PL finite in {fin:.4f} of the cells, eta != 2 eta_ini in {act:.4f} of the interior; "
+        f"eta in [{eta.min():.3e}, {eta.max():.3e}]")
+    assert fin == 1.0 and act > 0.99
+    worst = _compare_all(s, o)
+    say("worst rel-L2 over all fields", worst)
+    s.close()
+
+
+# ---------------------------------------------------------------- 511^3 plateau
+def test_config3_511_plateau_3_iterations():
+    st = setups.plateau_sector(511)
+    say("511^3 plateau: 3 fused iterations vs the oracle")
+    o = Oracle(st, copy=False)
+    s = _solver(st)
+    s.pt_iterate(st.params, 3)
+    t0 = time.time()
+    o.step(3)
+    say(f"  oracle done, {time.time() - t0:.0f} s")
+    _compare_all(s, o)
+    s.close()

@@ -37,3 +37,28 @@ def test_dump_restore_resumes_bit_for_bit(tmp_path):
     sys.stdout.flush()
     for n in STATE + ["eta_PL"]:
         assert np.array_equal(got[n], ref[n]), n
+
+
+def test_dump_matches_oracle_dump_byte_for_byte(tmp_path):
+    """The reference-format folder written from the GPU state and from the
+    oracle's state after the same 50 linear iterations (n_exp = 1, where the
+    iteration is bitwise) are byte-identical, file by file
+    (SphericalStokes_diapir.jl:23-43, :455-481)."""
+    from oracle.binding import Oracle
+    from sphericalstokes_amd.solver import Solver
+    st = setups.diapir_sector(23, nth=19, nph=29, n_exp=1.0)
+    p = st.params
+    o = Oracle(st)
+    o.step(50)
+    with Solver(st.nr, st.nth, st.nph) as s:
+        s.load_setup(st)
+        s.pt_iterate(p, 50)
+        io.save_reference_format(tmp_path / "gpu", s, st, err_evo=[1e-3, 2e-4], iter_evo=[100, 200])
+    io.save_reference_format(tmp_path / "cpu", o.f, st, err_evo=[1e-3, 2e-4], iter_evo=[100, 200])
+    names = sorted(q.name for q in (tmp_path / "gpu").iterdir())
+    assert names == sorted(q.name for q in (tmp_path / "cpu").iterdir())
+    assert len(names) >= 20
+    for name in names:
+        a, b = (tmp_path / "gpu" / name).read_bytes(), (tmp_path / "cpu" / name).read_bytes()
+        assert a == b, name
+    print("dumps identical:", len(names), "files", flush=True)

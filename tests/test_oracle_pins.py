@@ -205,7 +205,8 @@ def _last_bc_wins(V0, comp, mode, th, ph, r, sh):
                 if d == 1:
                     if comp == 1:
                         if mode == BC_PURE_SHEAR:
-                            vb = -(((th[0 if side == 0 else -1] - math.pi / 2) * r) * sh)
+                            tb = th[0 if side == 0 else -1]
+                            vb = -((((tb - math.pi / 2) * r) * sh) / math.sin(tb))
                             return 2.0 * vb - final(*q)
                         return -final(*q)
                     if mode == BC_SHEAR:
@@ -336,3 +337,148 @@ def test_hydrostatic_column():
     dPdr = (P[1:] - P[:-1]) * st.params._dr
     assert np.allclose(dPdr[:, 1:-1, 1:-1], -1.0, rtol=0, atol=1e-12)
     assert np.abs(o.f["VR"]).max() < 1e-14
+
+
+def test_pure_shear_boundary_fluxes_cancel():
+    """The pure-shear BC prescribes normal velocities whose fluxes through
+    the th and phi boundaries cancel (free slip closes the r faces), so a
+    divergence-free solution can exist.  The prescribed value on a face is
+    read back from the oracle's ghost/inner pair, (ghost + inner)/2, at every
+    tangential position, and integrated with the face area element
+    (R sin(th_b) dR dphi on a th face, R dR dth on a phi face)."""
+    st = setups.diapir_sector(9, nth=11, nph=13, n_exp=1.0, bc_mode=BC_PURE_SHEAR, sh=0.7)
+    rng = np.random.default_rng(5)
+    for k in ("VR", "Vth", "Vph"):
+        st.fields[k][...] = rng.standard_normal(st.fields[k].shape)
+    o = Oracle(st)
+    o.bc_R(); o.bc_th(); o.bc_ph()
+    VR, VT, VP = o.f["VR"], o.f["Vth"], o.f["Vph"]
+    R, th, ph = st.R, st.th, st.ph
+    def trapz(x):
+        w = np.gradient(x)
+        w[[0, -1]] *= 0.5
+        return w
+    wR, wT, wP = trapz(R) * R, trapz(th), trapz(ph)   # trapezoid rule over the boundary nodes
+    vr_lo, vr_hi = 0.5 * (VR[0] + VR[1]), 0.5 * (VR[-1] + VR[-2])
+    vt_lo, vt_hi = 0.5 * (VT[:, 0] + VT[:, 1]), 0.5 * (VT[:, -1] + VT[:, -2])      # (nr, nph)
+    vp_lo, vp_hi = 0.5 * (VP[:, :, 0] + VP[:, :, 1]), 0.5 * (VP[:, :, -1] + VP[:, :, -2])  # (nr, nth)
+    assert np.abs(vr_lo).max() < 1e-15 and np.abs(vr_hi).max() < 1e-15
+    # a uniform normal velocity on each face (the BC prescribes a constant)
+    for v in (vt_lo, vt_hi, vp_lo, vp_hi):
+        assert np.ptp(v) <= 1e-14 * np.abs(v).max()
+    flux_th = math.sin(th[-1]) * np.einsum("i,k,ik->", wR, wP, vt_hi) - math.sin(th[0]) * np.einsum("i,k,ik->", wR, wP, vt_lo)
+    flux_ph = np.einsum("i,j,ij->", wR, wT, vp_hi) - np.einsum("i,j,ij->", wR, wT, vp_lo)
+    assert abs(flux_th) > 0.1 and abs(flux_ph) > 0.1        # both faces carry flow
+    assert abs(flux_th + flux_ph) <= 1e-13 * abs(flux_ph), (flux_th, flux_ph)
+
+
+def test_pure_shear_seed_is_divergence_free():
+    """The seeded far-field pure shear (setups.pure_shear_velocity) is
+    divergence-free in the continuum, so compute_P!'s discrete divV of it is
+    O(h^2) and shrinks by about 4x per grid doubling."""
+    errs = []
+    for n in (9, 17, 33):
+        st = setups.diapir_sector(n, n_exp=1.0, bc_mode=BC_PURE_SHEAR, sh=1.0)
+        f = st.fields
+        for k in ("P", "tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph"):
+            f[k][...] = 0.0
+        f["etaSM"][...] = 1.0
+        f["eta"][...] = 1e300
+        o = Oracle(st)
+        o.compute_P()
+        p = st.params
+        kb = (((p.dmp * 2.0) * math.pi) * p.vpdt) / p.lr
+        errs.append(np.abs(o.f["P"] / kb).max())
+    assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5, errs
+    assert errs[-1] < 1e-3, errs
+
+
+def _check_err_brute(Vs, Eprev, drop_nan_vph):
+    comp = []
+    for V, E in zip(Vs, Eprev):
+        d = np.abs(V.ravel(order="F") - E.ravel(order="F"))
+        a = np.abs(V.ravel(order="F"))
+        comp.append(float("nan") if (np.isnan(d).any() or np.isnan(a).any()) else float(d.max() / a.max()))
+    cs = comp[:2] if (drop_nan_vph and math.isnan(comp[2])) else comp
+    return (float("nan") if any(math.isnan(c) for c in cs) else max(cs)), comp
+
+
+@pytest.mark.parametrize("case", ["plain", "nan_vph", "nan_vph_drop", "nan_vr"])
+def test_check_err_brute_force(case):
+    """check_err! + the host reduction (err = max over components of
+    max|V - V_prev| / max|V|, NaN-propagating; the plateau variant drops a
+    NaN Vph component) against a numpy brute force."""
+    st = setups.diapir(6, 7, 8, bc_mode=BC_FREE_SLIP)
+    rng = np.random.default_rng(3)
+    for k in ("VR", "Vth", "Vph"):
+        st.fields[k][...] = rng.standard_normal(st.fields[k].shape) * (1 + 3 * ("VR" == k))
+        st.fields[k + "_err"][...] = st.fields[k] + 1e-3 * rng.standard_normal(st.fields[k].shape)
+    if case.startswith("nan_vph"):
+        st.fields["Vph"][2, 3, 4] = np.nan
+    if case == "nan_vr":
+        st.fields["VR_err"][1, 1, 1] = np.nan
+    st.params.err_drop_nan_vph = int(case == "nan_vph_drop")
+    Vs = [st.fields[k].copy() for k in ("VR", "Vth", "Vph")]
+    Es = [st.fields[k + "_err"].copy() for k in ("VR", "Vth", "Vph")]
+    o = Oracle(st)
+    err, comp = o.check_err()
+    exp_err, exp_comp = _check_err_brute(Vs, Es, st.params.err_drop_nan_vph)
+    for a, b in zip(comp, exp_comp):
+        assert (math.isnan(a) and math.isnan(b)) or a == b
+    assert (math.isnan(err) and math.isnan(exp_err)) or err == exp_err
+    if case == "plain":
+        assert err == exp_comp[int(np.argmax(exp_comp))] and err > 0
+    # V_err <- V - V_prev
+    for k, V, E in zip(("VR", "Vth", "Vph"), Vs, Es):
+        assert np.array_equal(o.f[k + "_err"], V - E, equal_nan=True)
+
+
+def test_initialize_inclusion_brute_force():
+    """initialize_inclusion! tests the squared distance against `radius`
+    (SphericalStokes_diapir.jl:132): a point at distance d from the centre
+    is inside iff d^2 < radius, i.e. d < sqrt(radius).  Brute force over a
+    tiny grid with explicit Python loops, plus the quirk itself: radius 0.01
+    marks a sphere of radius 0.1."""
+    from oracle.binding import initialize_inclusion
+    st = setups.diapir_sector(11, nth=9, nph=13)
+    R, th, ph, r = st.R, st.th, st.ph, 1.5
+    for radius in (0.01, 0.05, 0.3):
+        A = np.full((11, 9, 13), 7.0, order="F")
+        initialize_inclusion(A, R, th, ph, r, radius, -2.5)
+        exp = np.full_like(A, 7.0)
+        for i in range(11):
+            for j in range(9):
+                for k in range(13):
+                    d = math.sqrt((R[i] - r) ** 2 + ((th[j] - math.pi / 2) * r) ** 2 + (ph[k] * r) ** 2)
+                    if d < math.sqrt(radius):
+                        exp[i, j, k] = -2.5
+        assert np.array_equal(A, exp), radius
+        # the numpy setup does the same
+        B = np.full_like(A, 7.0)
+        setups._inclusion(B, R, th, ph, r, radius, -2.5)
+        assert np.array_equal(B, exp), radius
+    A = np.zeros((11, 9, 13), order="F")
+    initialize_inclusion(A, R, th, ph, r, 0.01, 1.0)
+    inside = np.argwhere(A == 1.0)
+    dist = [math.sqrt((R[i] - r) ** 2 + ((th[j] - math.pi / 2) * r) ** 2 + (ph[k] * r) ** 2) for i, j, k in inside]
+    assert len(dist) > 0 and max(dist) < 0.1
+
+
+def test_initialize_velocity_brute_force():
+    """initialize_velocity!(Vph, th, sh, r) sets Vph = (th - pi/2) r sh on
+    every (nr, nth, nph+1) point (SphericalStokes_diapir.jl:139-146): a
+    simple shear, linear in th, zero on the equator."""
+    from oracle.binding import initialize_velocity
+    st = setups.diapir_sector(7, nth=9, nph=5)
+    th = st.th
+    V = np.full((7, 9, 6), 3.0, order="F")
+    initialize_velocity(V, th, 0.8, 1.5)
+    for j in range(9):
+        assert np.all(V[:, j, :] == (th[j] - math.pi / 2) * 1.5 * 0.8)
+    assert V[:, 4, :].max() == 0.0 and V[:, 4, :].min() == 0.0        # th = pi/2
+    slope = np.diff(V[0, :, 0]) / np.diff(th)
+    assert np.allclose(slope, 1.5 * 0.8, rtol=1e-12)
+    # the numpy setup's seed for simple shear is the same field
+    W = np.zeros_like(V)
+    setups._velocity_shear(W, th, 1.5, 0.8)
+    assert np.array_equal(W, V)
