@@ -183,39 +183,44 @@ void orc_compute_P(orc_state *s, const orc_params *p)
     /* tau_Rph: (nr-1, nth-2, nph-1) */
 #pragma omp parallel for schedule(static)
     for (int iz = 1; iz <= np - 1; ++iz)
-        for (int iy = 1; iy <= nt - 2; ++iy)
+        for (int iy = 1; iy <= nt - 2; ++iy) {
+            /* th-only terms evaluated once per (iy, iz) row: the same values */
+            const double Txz = (((T1[iy + 1] + T1[iy + 1]) + T1[iy + 1]) + T1[iy + 1]) * 0.25; /* @avxzi_th */
+            const double sTxz = sin(Txz);
             for (int ix = 1; ix <= nr - 1; ++ix) {
                 const double Rxz = (((R1[ix] + R1[ix]) + R1[ix + 1]) + R1[ix + 1]) * 0.25;   /* @avxzi_R */
-                const double Txz = (((T1[iy + 1] + T1[iy + 1]) + T1[iy + 1]) + T1[iy + 1]) * 0.25; /* @avxzi_th */
                 const double G = (((GS_(ix, iy + 1, iz) + GS_(ix, iy + 1, iz + 1)) + GS_(ix + 1, iy + 1, iz)) +
                                   GS_(ix + 1, iy + 1, iz + 1)) * 0.25;
                 const double E = (((C_(eta, ix, iy + 1, iz) + C_(eta, ix, iy + 1, iz + 1)) +
                                    C_(eta, ix + 1, iy + 1, iz)) + C_(eta, ix + 1, iy + 1, iz + 1)) * 0.25;
                 const double avVP = (VP_(ix, iy + 1, iz + 1) + VP_(ix + 1, iy + 1, iz + 1)) * 0.5; /* @avxi_Vph */
                 const double D = ((VP_(ix + 1, iy + 1, iz + 1) - VP_(ix, iy + 1, iz + 1)) * _dr +
-                                  (((1.0 / Rxz) / sin(Txz)) * (VR_(ix + 1, iy + 1, iz + 1) - VR_(ix + 1, iy + 1, iz))) * _dph) -
+                                  (((1.0 / Rxz) / sTxz) * (VR_(ix + 1, iy + 1, iz + 1) - VR_(ix + 1, iy + 1, iz))) * _dph) -
                                  avVP / Rxz;
                 const size_t e = IDX(nr - 1, nt - 2, ix, iy, iz);
                 tRP[e] = (tRP[e] + G * D) / (1.0 + G / E);
             }
+        }
     /* tau_thph: (nr-2, nth-1, nph-1) */
 #pragma omp parallel for schedule(static)
     for (int iz = 1; iz <= np - 1; ++iz)
-        for (int iy = 1; iy <= nt - 1; ++iy)
+        for (int iy = 1; iy <= nt - 1; ++iy) {
+            const double Tyz = (((T1[iy] + T1[iy]) + T1[iy + 1]) + T1[iy + 1]) * 0.25;         /* @avyzi_th */
+            const double sTyz = sin(Tyz), cTyz = 1.0 / tan(Tyz);
             for (int ix = 1; ix <= nr - 2; ++ix) {
                 const double Ryz = (((R1[ix + 1] + R1[ix + 1]) + R1[ix + 1]) + R1[ix + 1]) * 0.25; /* @avyzi_R */
-                const double Tyz = (((T1[iy] + T1[iy]) + T1[iy + 1]) + T1[iy + 1]) * 0.25;         /* @avyzi_th */
                 const double G = (((GS_(ix + 1, iy, iz) + GS_(ix + 1, iy, iz + 1)) + GS_(ix + 1, iy + 1, iz)) +
                                   GS_(ix + 1, iy + 1, iz + 1)) * 0.25;
                 const double E = (((C_(eta, ix + 1, iy, iz) + C_(eta, ix + 1, iy, iz + 1)) +
                                    C_(eta, ix + 1, iy + 1, iz)) + C_(eta, ix + 1, iy + 1, iz + 1)) * 0.25;
                 const double avVP = (VP_(ix + 1, iy, iz + 1) + VP_(ix + 1, iy + 1, iz + 1)) * 0.5; /* @avyi_Vph */
                 const double D = (((1.0 / Ryz) * (VP_(ix + 1, iy + 1, iz + 1) - VP_(ix + 1, iy, iz + 1))) * _dth +
-                                  (((1.0 / Ryz) / sin(Tyz)) * (VT_(ix + 1, iy + 1, iz + 1) - VT_(ix + 1, iy + 1, iz))) * _dph) -
-                                 (avVP / Ryz) * (1.0 / tan(Tyz));
+                                  (((1.0 / Ryz) / sTyz) * (VT_(ix + 1, iy + 1, iz + 1) - VT_(ix + 1, iy + 1, iz))) * _dph) -
+                                 (avVP / Ryz) * cTyz;
                 const size_t e = IDX(nr - 2, nt - 1, ix, iy, iz);
                 tTP[e] = (tTP[e] + G * D) / (1.0 + G / E);
             }
+        }
 #undef C_
 #undef VR_
 #undef VT_
@@ -298,10 +303,11 @@ void orc_compute_V(orc_state *s, const orc_params *p)
     /* VR on [2:nr, 2:nth-1, 2:nph-1] */
 #pragma omp parallel for schedule(static)
     for (int iz = 1; iz <= np - 2; ++iz)
-        for (int iy = 1; iy <= nt - 2; ++iy)
+        for (int iy = 1; iy <= nt - 2; ++iy) {
+            const double Tx = (T1[iy + 1] + T1[iy + 1]) * 0.5;      /* @avxi_th */
+            const double sTx = sin(Tx), cTx = 1.0 / tan(Tx);
             for (int ix = 1; ix <= nr - 1; ++ix) {
                 const double Rx = (R1[ix] + R1[ix + 1]) * 0.5;          /* @avxi_R */
-                const double Tx = (T1[iy + 1] + T1[iy + 1]) * 0.5;      /* @avxi_th */
                 const double sRR = (SRR(ix, iy + 1, iz + 1) + SRR(ix + 1, iy + 1, iz + 1)) * 0.5;
                 const double sTT = (STT(ix, iy + 1, iz + 1) + STT(ix + 1, iy + 1, iz + 1)) * 0.5;
                 const double sPP = (SPP(ix, iy + 1, iz + 1) + SPP(ix + 1, iy + 1, iz + 1)) * 0.5;
@@ -309,52 +315,57 @@ void orc_compute_V(orc_state *s, const orc_params *p)
                 const double rg = (C_(rho, ix, iy + 1, iz + 1) * g + C_(rho, ix + 1, iy + 1, iz + 1) * g) * 0.5;
                 const double dV = (((((((SRR(ix + 1, iy + 1, iz + 1) - SRR(ix, iy + 1, iz + 1)) * _dr +
                                         ((1.0 / Rx) * (RT_(ix, iy + 1, iz) - RT_(ix, iy, iz))) * _dth) +
-                                       (((1.0 / Rx) / sin(Tx)) * (RP_(ix, iy, iz + 1) - RP_(ix, iy, iz))) * _dph) +
+                                       (((1.0 / Rx) / sTx) * (RP_(ix, iy, iz + 1) - RP_(ix, iy, iz))) * _dph) +
                                       (2.0 * sRR) / Rx) -
                                      (sTT + sPP) / Rx) +
-                                    (aRT / Rx) * (1.0 / tan(Tx))) -
+                                    (aRT / Rx) * cTx) -
                                    rg);
                 const double dt = (C_(DT, ix, iy + 1, iz + 1) + C_(DT, ix + 1, iy + 1, iz + 1)) * 0.5;
                 const size_t f = IDX(nr + 1, nt, ix + 1, iy + 1, iz + 1);
                 VR[f] = VR[f] + dV * dt;
             }
+        }
     /* Vth on [2:nr-1, 2:nth, 2:nph-1] */
 #pragma omp parallel for schedule(static)
     for (int iz = 1; iz <= np - 2; ++iz)
-        for (int iy = 1; iy <= nt - 1; ++iy)
+        for (int iy = 1; iy <= nt - 1; ++iy) {
+            const double Ty = (T1[iy] + T1[iy + 1]) * 0.5;          /* @avyi_th */
+            const double sTy = sin(Ty), cTy = 1.0 / tan(Ty);
             for (int ix = 1; ix <= nr - 2; ++ix) {
                 const double Ry = (R1[ix + 1] + R1[ix + 1]) * 0.5;      /* @avyi_R */
-                const double Ty = (T1[iy] + T1[iy + 1]) * 0.5;          /* @avyi_th */
                 const double sTT = (STT(ix + 1, iy, iz + 1) + STT(ix + 1, iy + 1, iz + 1)) * 0.5;
                 const double sPP = (SPP(ix + 1, iy, iz + 1) + SPP(ix + 1, iy + 1, iz + 1)) * 0.5;
                 const double aRT = (RT_(ix, iy, iz) + RT_(ix + 1, iy, iz)) * 0.5; /* @avxa_tauRth */
                 const double dV = ((((RT_(ix + 1, iy, iz) - RT_(ix, iy, iz)) * _dr +
                                      ((1.0 / Ry) * (STT(ix + 1, iy + 1, iz + 1) - STT(ix + 1, iy, iz + 1))) * _dth) +
-                                    (((1.0 / Ry) / sin(Ty)) * (TP_(ix, iy, iz + 1) - TP_(ix, iy, iz))) * _dph) +
+                                    (((1.0 / Ry) / sTy) * (TP_(ix, iy, iz + 1) - TP_(ix, iy, iz))) * _dph) +
                                    (3.0 * aRT) / Ry) +
-                                  ((sTT - sPP) / Ry) * (1.0 / tan(Ty));
+                                  ((sTT - sPP) / Ry) * cTy;
                 const double dt = (C_(DT, ix + 1, iy, iz + 1) + C_(DT, ix + 1, iy + 1, iz + 1)) * 0.5;
                 const size_t f = IDX(nr, nt + 1, ix + 1, iy + 1, iz + 1);
                 VT[f] = VT[f] + dV * dt;
             }
+        }
     /* Vph on [2:nr-1, 2:nth-1, 2:nph] */
 #pragma omp parallel for schedule(static)
     for (int iz = 1; iz <= np - 1; ++iz)
-        for (int iy = 1; iy <= nt - 2; ++iy)
+        for (int iy = 1; iy <= nt - 2; ++iy) {
+            const double Tz = (T1[iy + 1] + T1[iy + 1]) * 0.5;      /* @avzi_th */
+            const double sTz = sin(Tz), cTz = 1.0 / tan(Tz);
             for (int ix = 1; ix <= nr - 2; ++ix) {
                 const double Rz = (R1[ix + 1] + R1[ix + 1]) * 0.5;      /* @avzi_R */
-                const double Tz = (T1[iy + 1] + T1[iy + 1]) * 0.5;      /* @avzi_th */
                 const double aRP = (RP_(ix, iy, iz) + RP_(ix + 1, iy, iz)) * 0.5; /* @avxa_tauRph */
                 const double aTP = (TP_(ix, iy, iz) + TP_(ix, iy + 1, iz)) * 0.5; /* @avya_tauthph */
                 const double dV = ((((RP_(ix + 1, iy, iz) - RP_(ix, iy, iz)) * _dr +
                                      ((1.0 / Rz) * (TP_(ix, iy + 1, iz) - TP_(ix, iy, iz))) * _dth) +
-                                    (((1.0 / Rz) / sin(Tz)) * (SPP(ix + 1, iy + 1, iz + 1) - SPP(ix + 1, iy + 1, iz))) * _dph) +
+                                    (((1.0 / Rz) / sTz) * (SPP(ix + 1, iy + 1, iz + 1) - SPP(ix + 1, iy + 1, iz))) * _dph) +
                                    (3.0 * aRP) / Rz) +
-                                  ((2.0 * aTP) / Rz) * (1.0 / tan(Tz));
+                                  ((2.0 * aTP) / Rz) * cTz;
                 const double dt = (C_(DT, ix + 1, iy + 1, iz) + C_(DT, ix + 1, iy + 1, iz + 1)) * 0.5;
                 const size_t f = IDX(nr, nt, ix + 1, iy + 1, iz + 1);
                 VP[f] = VP[f] + dV * dt;
             }
+        }
 #undef C_
 #undef SRR
 #undef STT

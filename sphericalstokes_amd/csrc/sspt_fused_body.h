@@ -76,23 +76,33 @@ SSPT_DH inline uint32_t tbp_off(const BoxG &b, int side, int i, int j)
     return ((uint32_t)side * (uint32_t)b.nr + (uint32_t)i) * (uint32_t)b.nt + (uint32_t)j;
 }
 
+// SSPT_ACCESS_CHECK(base, byte_offset, bytes): a hook on every field access
+// through the 32-bit offsets below; empty in the library, a bounds check in
+// the host emulation's sanitizer build.
+#ifndef SSPT_ACCESS_CHECK
+#define SSPT_ACCESS_CHECK(p, off, n) ((void)0)
+#endif
+
 // Element o of a field through a 32-bit byte offset from the field's base:
 // with a wave-uniform base this is the SGPR-base + 32-bit VGPR-offset form of
 // global memory instructions, with no 64-bit address arithmetic per access.
 template <typename T> SSPT_DH inline T &fat(T *p, uint32_t o)
 {
+    SSPT_ACCESS_CHECK(p, (uint32_t)(o * (uint32_t)sizeof(T)), sizeof(T));
     return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + (uint32_t)(o * (uint32_t)sizeof(T)));
 }
 // Element o of a field at byte offset `off` from base p (the in or out half
 // of a paired allocation; off + 8 o stays below 4 GiB).
 template <typename T> SSPT_DH inline const T &fat_pp(const T *p, uint32_t o, uint32_t off)
 {
+    SSPT_ACCESS_CHECK(p, (uint32_t)(o * (uint32_t)sizeof(T) + off), sizeof(T));
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint32_t)(o * (uint32_t)sizeof(T) + off));
 }
 // the in half of a state field (offset first, for reading at call sites)
 template <typename T> SSPT_DH inline const T &fat_in(const T *p, uint32_t off, uint32_t o) { return fat_pp(p, o, off); }
 template <typename T> SSPT_DH inline const T &fat(const T *p, uint32_t o)
 {
+    SSPT_ACCESS_CHECK(p, (uint32_t)(o * (uint32_t)sizeof(T)), sizeof(T));
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (uint32_t)(o * (uint32_t)sizeof(T)));
 }
 

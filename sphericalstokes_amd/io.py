@@ -117,3 +117,135 @@ def restore(solver, fields) -> None:
         f["eta_PL"] = f["eta_ini"]
     for name, a in f.items():
         solver.upload(name, a)
+
+
+# ---------------------------------------------------------------- XDMF output
+# The reference's visualisation output (scripts_perf/SphericalStokes_mxpu.jl
+# :443-464, writers in scripts_perf/data_io2.jl:1-132): per save, the inner
+# cells' Cartesian coordinates X, Y, Z (sph2cart!, SphericalStokes_mxpu.jl
+# :32-38) and fields averaged onto the inner cell centres, described by an
+# XDMF 3.0 file as a temporal collection of 3DSMesh grids with cell-centred
+# attributes.  The reference stores the arrays in one HDF5 file per save;
+# this build has no HDF5 library, so each save is one raw little-endian FP64
+# file holding the datasets back to back, and every XDMF DataItem points
+# into it with Format="Binary" and a byte Seek offset.  Dataset names,
+# order of dimensions and the XML layout are the reference's.
+XDMF_FIELDS = ["eta", "P", "tII", "VR", "VT", "VF", "t_RR", "t_TT", "t_FF", "t_RT", "τ_RF", "τ_TF"]
+
+
+def sph2cart(R, th, ph):
+    """X, Y, Z of the points (R_i, th_j, ph_k) as the reference maps them
+    (sph2cart!, SphericalStokes_mxpu.jl:32-38; the pi/2 - th is specific to
+    its sector about the equator): X = R cos(pi/2-th) cos(ph),
+    Y = R sin(pi/2-th) cos(ph), Z = R sin(ph).  3D arrays (Julia order)."""
+    R3 = np.asarray(R, dtype=np.float64)[:, None, None]
+    T3 = np.asarray(th, dtype=np.float64)[None, :, None]
+    P3 = np.asarray(ph, dtype=np.float64)[None, None, :]
+    X = R3 * np.cos(np.pi / 2 - T3) * np.cos(P3)
+    Y = R3 * np.sin(np.pi / 2 - T3) * np.cos(P3)
+    Z = np.broadcast_to(R3 * np.sin(P3), X.shape)
+    return X, Y, np.ascontiguousarray(Z)
+
+
+def _inn(A):
+    return A[1:-1, 1:-1, 1:-1]
+
+
+def xdmf_datasets(source, setup) -> dict:
+    """The reference's save set (SphericalStokes_mxpu.jl:450-452) on the
+    (nr-2, nth-2, nph-2) inner cells: X, Y, Z, then eta, P, tII, the
+    velocities averaged from their faces (av_x2i/av_y2i/av_z2i) and the
+    stresses (inner centres; shear stresses averaged from their 4 edges,
+    av_xya/av_xza/av_yza), as Fortran-ordered float64 arrays."""
+    get = source.download if hasattr(source, "download") else source.__getitem__
+    X, Y, Z = sph2cart(setup.R[1:-1], setup.th[1:-1], setup.ph[1:-1])
+    VR, VT, VP = get("VR"), get("Vth"), get("Vph")
+    RT, RP, TP = get("tau_Rth"), get("tau_Rph"), get("tau_thph")
+    d = {"X": X, "Y": Y, "Z": Z,
+         "eta": _inn(get("eta")), "P": _inn(get("P")), "tII": _inn(get("tII")),
+         "VR": (VR[2:-1, 1:-1, 1:-1] + VR[1:-2, 1:-1, 1:-1]) * 0.5,
+         "VT": (VT[1:-1, 2:-1, 1:-1] + VT[1:-1, 1:-2, 1:-1]) * 0.5,
+         "VF": (VP[1:-1, 1:-1, 2:-1] + VP[1:-1, 1:-1, 1:-2]) * 0.5,
+         "t_RR": _inn(get("tau_RR")), "t_TT": _inn(get("tau_thth")), "t_FF": _inn(get("tau_phph")),
+         "t_RT": (((RT[:-1, :-1, :] + RT[1:, :-1, :]) + RT[:-1, 1:, :]) + RT[1:, 1:, :]) * 0.25,
+         "τ_RF": (((RP[:-1, :, :-1] + RP[1:, :, :-1]) + RP[:-1, :, 1:]) + RP[1:, :, 1:]) * 0.25,
+         "τ_TF": (((TP[:, :-1, :-1] + TP[:, 1:, :-1]) + TP[:, :-1, 1:]) + TP[:, 1:, 1:]) * 0.25}
+    return {k: np.asfortranarray(v, dtype=np.float64) for k, v in d.items()}
+
+
+class XdmfSeries:
+    """A time series of saves: ``add(source, setup, t)`` writes
+    ``<name>_<isave>.bin`` and rewrites ``<name>.xdmf`` over all saves so
+    far (the reference's write_xdmf with h5_names and timesteps)."""
+
+    def __init__(self, out, name: str = "results3D"):
+        self.out = Path(out)
+        self.out.mkdir(parents=True, exist_ok=True)
+        self.name = name
+        self.saves = []   # (file name, time, dims, [(dataset, seek)])
+
+    def add(self, source, setup, t: float) -> Path:
+        d = xdmf_datasets(source, setup)
+        dims = d["X"].shape
+        fname = f"{self.name}_{len(self.saves) + 1}.bin"
+        items, seek = [], 0
+        with open(self.out / fname, "wb") as fh:
+            for key in ["X", "Y", "Z"] + XDMF_FIELDS:
+                a = d[key]
+                assert a.shape == dims, (key, a.shape, dims)
+                fh.write(a.astype("<f8").tobytes(order="F"))
+                items.append((key, seek))
+                seek += a.size * 8
+        self.saves.append((fname, float(t), dims, items))
+        return self.write_xdmf()
+
+    def write_xdmf(self) -> Path:
+        import xml.etree.ElementTree as ET
+        root = ET.Element("Xdmf", Version="3.0")
+        dom = ET.SubElement(root, "Domain")
+        coll = ET.SubElement(dom, "Grid", GridType="Collection", CollectionType="Temporal")
+
+        def item(parent, fname, dims, seek, **kw):
+            di = ET.SubElement(parent, "DataItem", **kw, Format="Binary", NumberType="Float", Precision="8",
+                               Endian="Little", Seek=str(seek), Dimensions=" ".join(str(n) for n in reversed(dims)))
+            di.text = fname
+            return di
+
+        for fname, t, dims, items in self.saves:
+            g = ET.SubElement(coll, "Grid", GridType="Uniform")
+            ET.SubElement(g, "Topology", TopologyType="3DSMesh", Dimensions=" ".join(str(n) for n in reversed(dims)))
+            ET.SubElement(g, "Time", Value=repr(t))
+            geo = ET.SubElement(g, "Geometry", GeometryType="X_Y_Z")
+            seeks = dict(items)
+            for c in ("X", "Y", "Z"):
+                item(geo, fname, dims, seeks[c], Name=c)
+            for key in XDMF_FIELDS:
+                att = ET.SubElement(g, "Attribute", Name=key, Center="Cell")
+                item(att, fname, dims, seeks[key])
+        path = self.out / f"{self.name}.xdmf"
+        ET.ElementTree(root).write(path, encoding="utf-8", xml_declaration=True)
+        return path
+
+
+def read_xdmf_dataitem(xdmf_path, grid_index: int, name: str) -> np.ndarray:
+    """Reads one dataset of a save back through its XDMF description
+    (file, Seek, Dimensions): the geometry's X/Y/Z or an attribute."""
+    import xml.etree.ElementTree as ET
+    xdmf_path = Path(xdmf_path)
+    grids = ET.parse(xdmf_path).getroot().find("Domain").find("Grid").findall("Grid")
+    g = grids[grid_index]
+    di = None
+    for d in g.find("Geometry").findall("DataItem"):
+        if d.get("Name") == name:
+            di = d
+    for a in g.findall("Attribute"):
+        if a.get("Name") == name:
+            di = a.find("DataItem")
+    if di is None:
+        raise KeyError(name)
+    dims = tuple(int(x) for x in di.get("Dimensions").split())[::-1]
+    n = int(np.prod(dims))
+    with open(xdmf_path.parent / di.text.strip(), "rb") as fh:
+        fh.seek(int(di.get("Seek")))
+        a = np.frombuffer(fh.read(8 * n), dtype="<f8")
+    return np.asfortranarray(a.reshape(dims, order="F"))

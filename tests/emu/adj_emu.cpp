@@ -6,6 +6,10 @@
 // BC fold and the boundary power-law sums that the
 // library computes in separate small kernels are restated here as loops.
 #define SSPT_DH
+#ifndef SSPT_ACCESS_REGISTER
+#define SSPT_ACCESS_REGISTER(p, n) ((void)0)
+#define SSPT_ACCESS_UNREGISTER(p) ((void)0)
+#endif
 #define SSPT_UNIFORM(x) (x)
 #define SSPT_CONSTANT_PTR(T, p) (p)
 #include "../../sphericalstokes_amd/csrc/sspt_adj_fused_body.h"
@@ -129,6 +133,7 @@ int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const doub
     bc_fold(b, p->bc_mode, lam_in[A_VR], lam_in[A_VT], lam_in[A_VP]);
     const size_t nbox = (size_t)(nr + 1) * (nt + 1) * pitch;
     std::vector<double> LB(nbox, std::nan(""));
+    SSPT_ACCESS_REGISTER(LB.data(), LB.size() * sizeof(double));
     for (int i = 1; i <= nr - 2; ++i)
         for (int j = 1; j <= nt - 2; ++j)
             for (int k = 1; k <= np - 2; ++k)
@@ -170,6 +175,7 @@ int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const doub
                     for (auto &T : th) aphaseC(T, *L, a, q, wr, wr_m);
                 }
             }
+    SSPT_ACCESS_UNREGISTER(LB.data());
     return 0;
 }
 

@@ -7,6 +7,10 @@
 // fused schedule (tiling, halos, r-march, carries, boundary pass) against the
 // CPU oracle without a GPU.  It is never part of the product library.
 #define SSPT_DH
+#ifndef SSPT_ACCESS_REGISTER
+#define SSPT_ACCESS_REGISTER(p, n) ((void)0)
+#define SSPT_ACCESS_UNREGISTER(p) ((void)0)
+#endif
 #define SSPT_UNIFORM(x) (x)
 #define SSPT_CONSTANT_PTR(T, p) (p)
 #include "../../sphericalstokes_amd/csrc/sspt_fused_body.h"
@@ -155,6 +159,7 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
     for (int c = 0; c < 6; ++c) red[c] = 0ull;
     double **in = cur, **out = alt;
     std::vector<double> tbp(2 * (size_t)nr * nt, std::nan(""));
+    SSPT_ACCESS_REGISTER(tbp.data(), tbp.size() * sizeof(double));
     int which = 0;
     for (int it = 0; it < niter; ++it) {
         const bool last = it == niter - 1, check = last && check_last;
@@ -185,12 +190,16 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
         a.vth1 = pure_shear_vth(th1[nt - 1], p->r, p->sh);
         a.rchunk = rchunk > 0 ? rchunk : nr;
         Args<double> am = a;
-        if (pair_bytes > 0 && !pair_args(am, pair_bytes)) return -1;
+        if (pair_bytes > 0 && !pair_args(am, pair_bytes)) {
+            SSPT_ACCESS_UNREGISTER(tbp.data());
+            return -1;
+        }
         run_main(am, a.rchunk, red);
         run_boundary(a, red);
         std::swap(in, out);
         which ^= 1;
     }
+    SSPT_ACCESS_UNREGISTER(tbp.data());
     return which;
 }
 
