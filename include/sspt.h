@@ -70,6 +70,12 @@ typedef struct ss_params {
     double eta_min, eta_max; /* viscosity cutoffs; 0 / +inf disable */
     int bc_mode;             /* ss_bc_mode */
     int err_drop_nan_vph;    /* plateau convergence test ignores a NaN Vph error */
+    int etasm_spread;        /* 0: etaSM = maxloc! + copy_BC! as the reference (SphericalStokes_diapir.jl:149-152).
+                                m in 1..8: build option for sharp viscosity contrasts above ~1e3, where the
+                                reference's step sizes diverge -- the maxloc'd etaSM is then spread by three
+                                separable passes (r, th, phi) of out(i) = max_{|a|<=m} in(clamp(i+a)) * 0.1^|a|,
+                                so the pseudo-time steps of neighbouring cells differ by at most 10x near a jump.
+                                The converged solution does not depend on it. */
 } ss_params;
 
 typedef struct ss_result {
@@ -109,6 +115,11 @@ int ss_initialize_velocity(ss_ctx *ctx, double fact, double r);
 /* ---- per-kernel entry points (one per reference kernel) ---- */
 int ss_equal3(ss_ctx *ctx);                                   /* equal3!       :46-58   */
 int ss_maxloc(ss_ctx *ctx);                                   /* maxloc!       :149-152 */
+/* The etaSM spread of ss_params.etasm_spread = m (1..8) applied to etaSM in
+ * place, for the per-kernel sequence after maxloc! + copy_BC!(etaSM).  No
+ * reference counterpart (build option); DT_R serves as its workspace, which
+ * compute_P! rewrites next in the sequence. */
+int ss_spread_etasm(ss_ctx *ctx, int m);
 int ss_copy_bc(ss_ctx *ctx, int field);                       /* copy_BC!      :102-124 */
 int ss_compute_P(ss_ctx *ctx, double _dr, double _dth, double _dph,
                  double dmp, double vpdt, double lr);         /* compute_P!    :209-241 */

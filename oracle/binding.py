@@ -32,7 +32,7 @@ class orc_params(C.Structure):
                 ("vpdt", C.c_double), ("lr", C.c_double), ("tau_C", C.c_double), ("n_exp", C.c_double),
                 ("g", C.c_double), ("r", C.c_double), ("sh", C.c_double), ("relax", C.c_double),
                 ("eta_min", C.c_double), ("eta_max", C.c_double), ("bc_mode", C.c_int),
-                ("err_drop_nan_vph", C.c_int)]
+                ("err_drop_nan_vph", C.c_int), ("etasm_spread", C.c_int)]
 
 
 # adjoint state, order of the ORC_A_* enum (oracle/sspt_adjoint.h)
@@ -59,6 +59,7 @@ def lib():
         sig = {
             "orc_equal3": (None, [DP, DP, DP, DP, DP, DP, I, I, I]),
             "orc_maxloc": (None, [DP, DP, I, I, I]),
+            "orc_spread_etasm": (None, [DP, DP, I, I, I, I]),
             "orc_copy_bc": (None, [DP, I, I, I]),
             "orc_compute_P": (None, [SP, PP]),
             "orc_compute_TII": (None, [SP]),
@@ -93,7 +94,7 @@ def _ptr(a):
 def make_params(p) -> orc_params:
     return orc_params(p._dr, p._dth, p._dph, p.dmp, p.vpdt, p.lr, p.tau_C, p.n_exp, p.g, p.r, p.sh, p.relax,
                       p.eta_min, p.eta_max if p.eta_max is not None else math.inf, int(p.bc_mode),
-                      int(p.err_drop_nan_vph))
+                      int(p.err_drop_nan_vph), int(getattr(p, "etasm_spread", 0)))
 
 
 class Oracle:

@@ -91,6 +91,41 @@ void orc_maxloc(double *etaSM, const double *eta, int nr, int nth, int nph)
             }
 }
 
+/* Build option without a reference counterpart (etasm_spread = m): after
+ * maxloc! + copy_BC!, etaSM is spread by three separable passes r, th, phi
+ * of out(i) = max over |a| <= m of in(clamp(i + a)) * w[|a|], w[0] = 1,
+ * w[a] = w[a-1] * 0.1, so that the pseudo-time steps of neighbouring cells
+ * differ by at most a factor 10 within m cells of a stiff cell.  Pinned by
+ * brute force (tests/test_etasm_spread.py), not by a reference. */
+static inline int clampi_(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+void orc_spread_etasm(double *A, double *tmp, int nr, int nth, int nph, int m)
+{
+    double w[9];
+    w[0] = 1.0;
+    for (int a = 1; a <= 8; ++a) w[a] = w[a - 1] * 0.1;
+    const int n[3] = {nr, nth, nph};
+    for (int axis = 0; axis < 3; ++axis) {
+        const double *src = axis == 1 ? tmp : A;
+        double *dst = axis == 1 ? A : tmp;
+#pragma omp parallel for schedule(static)
+        for (int iz = 1; iz <= nph; ++iz)
+            for (int iy = 1; iy <= nth; ++iy)
+                for (int ix = 1; ix <= nr; ++ix) {
+                    int idx[3] = {ix, iy, iz};
+                    const int c = idx[axis];
+                    double v = src[IDX(nr, nth, ix, iy, iz)];
+                    for (int a = 1; a <= m; ++a) {
+                        idx[axis] = clampi_(c - a, 1, n[axis]);
+                        v = jl_max(v, src[IDX(nr, nth, idx[0], idx[1], idx[2])] * w[a]);
+                        idx[axis] = clampi_(c + a, 1, n[axis]);
+                        v = jl_max(v, src[IDX(nr, nth, idx[0], idx[1], idx[2])] * w[a]);
+                    }
+                    dst[IDX(nr, nth, ix, iy, iz)] = v;
+                }
+    }
+    memcpy(A, tmp, (size_t)nr * nth * nph * sizeof(double));
+}
+
 /* copy_bc_r!, copy_bc_th!, copy_bc_ph! in that order,
  * SphericalStokes_diapir.jl:102-124 */
 void orc_copy_bc(double *A, int n1, int n2, int n3)
@@ -573,6 +608,9 @@ void orc_pt_step(orc_state *s, const orc_params *p, int do_equal3)
     if (do_equal3) orc_equal3(s->VR_err, s->VT_err, s->VP_err, s->VR, s->VT, s->VP, nr, nt, np);
     orc_maxloc(s->etaSM, s->eta, nr, nt, np);
     orc_copy_bc(s->etaSM, nr, nt, np);
+    /* build option (no reference counterpart): DT_R serves as the
+     * workspace, compute_P! overwrites every entry of it next */
+    if (p->etasm_spread > 0) orc_spread_etasm(s->etaSM, s->DT_R, nr, nt, np, p->etasm_spread);
     orc_compute_P(s, p);
     orc_compute_TII(s);
     orc_copy_bc(s->tII, nr, nt, np);

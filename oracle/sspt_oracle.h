@@ -47,6 +47,7 @@ typedef struct orc_params {
     double eta_min, eta_max;/* cutoffs; <=0 / inf disable */
     int bc_mode;            /* ORC_BC_* */
     int err_drop_nan_vph;   /* plateau: ignore NaN err of Vph */
+    int etasm_spread;       /* 0 = reference maxloc!; m > 0: spread (orc_spread_etasm) */
 } orc_params;
 
 /* --- one function per reference kernel --- */
@@ -54,6 +55,10 @@ void orc_equal3(double *VR_err, double *VT_err, double *VP_err,
                 const double *VR, const double *VT, const double *VP,
                 int nr, int nth, int nph);
 void orc_maxloc(double *etaSM, const double *eta, int nr, int nth, int nph);
+/* build option etasm_spread = m: etaSM <- three separable passes (r, th, phi) of
+ * out(i) = max_{|a|<=m} in(clamp(i+a)) * w[|a|], w[0] = 1, w[a] = w[a-1] * 0.1;
+ * tmp is a centre-shaped workspace */
+void orc_spread_etasm(double *etaSM, double *tmp, int nr, int nth, int nph, int m);
 void orc_copy_bc(double *A, int n1, int n2, int n3);
 void orc_compute_P(orc_state *s, const orc_params *p);
 void orc_compute_TII(orc_state *s);

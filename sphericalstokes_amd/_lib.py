@@ -23,14 +23,14 @@ class ss_params(C.Structure):
         ("tau_C", C.c_double), ("n_exp", C.c_double), ("g", C.c_double),
         ("r", C.c_double), ("sh", C.c_double), ("relax", C.c_double),
         ("eta_min", C.c_double), ("eta_max", C.c_double),
-        ("bc_mode", C.c_int), ("err_drop_nan_vph", C.c_int),
+        ("bc_mode", C.c_int), ("err_drop_nan_vph", C.c_int), ("etasm_spread", C.c_int),
     ]
 
     @classmethod
     def from_params(cls, p) -> "ss_params":
         return cls(p._dr, p._dth, p._dph, p.dmp, p.vpdt, p.lr, p.tau_C, p.n_exp, p.g, p.r, p.sh,
                    p.relax, p.eta_min, p.eta_max if p.eta_max is not None else math.inf,
-                   int(p.bc_mode), int(p.err_drop_nan_vph))
+                   int(p.bc_mode), int(p.err_drop_nan_vph), int(getattr(p, "etasm_spread", 0)))
 
 
 class ss_result(C.Structure):
@@ -81,6 +81,7 @@ SIGNATURES = {
     "ss_initialize_velocity": (I, [P, D, D]),
     "ss_equal3": (I, [P]),
     "ss_maxloc": (I, [P]),
+    "ss_spread_etasm": (I, [P, I]),
     "ss_copy_bc": (I, [P, I]),
     "ss_compute_P": (I, [P, D, D, D, D, D, D]),
     "ss_compute_TII": (I, [P]),

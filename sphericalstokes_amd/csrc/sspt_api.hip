@@ -385,6 +385,15 @@ int ss_maxloc(ss_ctx *c)
     return sync(c, "ss_maxloc");
 }
 
+int ss_spread_etasm(ss_ctx *c, int m)
+{
+    if (!c) return SS_ERR_ARG;
+    if (m < 1 || m > SS_SPREAD_MAX) return fail(c, SS_ERR_ARG, "ss_spread_etasm: m must be in [1, 8]");
+    HIP_TRY(c, hipSetDevice(c->device));
+    launch_etasm_spread(c, c->cur.f[SS_ETASM], c->cur.f[SS_DT_R], m);
+    return sync(c, "ss_spread_etasm");
+}
+
 int ss_copy_bc(ss_ctx *c, int field)
 {
     if (!c) return SS_ERR_ARG;
@@ -494,6 +503,8 @@ int validate_params(ss_ctx *c, const ss_params *p)
     if (!p) return fail(c, SS_ERR_ARG, "null params");
     if (p->bc_mode < SS_BC_SHEAR || p->bc_mode > SS_BC_PURE_SHEAR) return fail(c, SS_ERR_ARG, "bad bc_mode");
     if (!(p->relax > 0.0 && p->relax <= 1.0)) return fail(c, SS_ERR_ARG, "relax must be in (0,1]");
+    if (p->etasm_spread < 0 || p->etasm_spread > SS_SPREAD_MAX)
+        return fail(c, SS_ERR_ARG, "etasm_spread must be in [0, 8]");
     return SS_OK;
 }
 
@@ -506,6 +517,8 @@ void unfused_step(ss_ctx *c, const ss_params *p, bool do_equal3)
     if (do_equal3) launch_equal3(c, s);
     launch_maxloc(c, s.f[SS_ETASM], s.f[SS_ETA]);
     launch_copy_bc(c, s.f[SS_ETASM], b.nr, b.nt, b.np);
+    // spread etaSM (build option); DT_R is the temporary, compute_P! rewrites all of it next
+    if (p->etasm_spread > 0) launch_etasm_spread(c, s.f[SS_ETASM], s.f[SS_DT_R], p->etasm_spread);
     launch_compute_P(c, s, p->_dr, p->_dth, p->_dph, p->dmp, p->vpdt, p->lr);
     launch_compute_TII(c, s);
     launch_copy_bc(c, s.f[SS_TII], b.nr, b.nt, b.np);

@@ -137,6 +137,54 @@ __global__ void __launch_bounds__(256) k_fused_boundary(Args<R> a)
     if (a.flags & F_CHECK) flush_max(a, mdv, mv, (threadIdx.x + threadIdx.y * blockDim.x) & 63);
 }
 
+// dst = copy_BC!(maxloc!(src)) over every cell (star_bc_at).
+template <typename R> __global__ void __launch_bounds__(256) k_star_bc(BoxG b, const R *__restrict__ src, R *__restrict__ dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k < b.np && j < b.nt && i < b.nr) dst[b.off(i, j, k)] = star_bc_at(b, src, i, j, k);
+}
+
+// One separable pass of the etaSM spread (spread_at) over every cell.
+template <typename R>
+__global__ void __launch_bounds__(256) k_spread(BoxG b, const R *__restrict__ src, R *__restrict__ dst, int axis, int m)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    const SpreadW<R> w = spread_weights<R>();
+    if (k < b.np && j < b.nt && i < b.nr) dst[b.off(i, j, k)] = spread_at(b, src, i, j, k, axis, m, w);
+}
+
+// With eta != null: sm <- spread(maxloc! + copy_BC! of eta) in four passes
+// (star: eta -> tmp, r: tmp -> sm, th: sm -> tmp, phi: tmp -> sm).  With
+// eta == null: sm <- spread(sm) (r: sm -> tmp, th: tmp -> sm, phi: sm ->
+// tmp, then tmp copied back).
+template <typename R> void launch_etasm_spread_box(ss_ctx *c, const BoxG &b, const R *eta, R *sm, R *tmp, int m)
+{
+    const dim3 G((b.np + 63) / 64, (b.nt + 3) / 4, b.nr), B(64, 4);
+    R *x = sm, *y = tmp;
+    if (eta) {
+        hipLaunchKernelGGL((k_star_bc<R>), G, B, 0, c->stream, b, eta, tmp);
+        x = tmp;
+        y = sm;
+    }
+    hipLaunchKernelGGL((k_spread<R>), G, B, 0, c->stream, b, (const R *)x, y, 0, m);
+    hipLaunchKernelGGL((k_spread<R>), G, B, 0, c->stream, b, (const R *)y, x, 1, m);
+    hipLaunchKernelGGL((k_spread<R>), G, B, 0, c->stream, b, (const R *)x, y, 2, m);
+    if (!eta)
+        (void)hipMemcpyAsync(sm, tmp, (size_t)(b.nr + 1) * (size_t)b.si * sizeof(R), hipMemcpyDeviceToDevice,
+                             c->stream);
+}
+template void launch_etasm_spread_box<float>(ss_ctx *, const BoxG &, const float *, float *, float *, int);
+
+void launch_etasm_spread(ss_ctx *c, double *SM, double *tmp, int m)
+{
+    const BoxG b{c->box.nr, c->box.nt, c->box.np, c->box.sj, c->box.si};
+    launch_etasm_spread_box<double>(c, b, nullptr, SM, tmp, m);
+}
+
 __global__ void k_zero_red(unsigned long long *p)
 {
     if (threadIdx.x < 6) p[threadIdx.x] = 0ull;
@@ -199,7 +247,8 @@ static Args<double> make_args(ss_ctx *c, const FieldSet &in, const FieldSet &out
     a.emin = p.eta_min;
     a.emax = p.eta_max;
     a.flags = (last ? F_LAST : 0) | (check ? F_CHECK : 0) | (p.relax != 1.0 ? F_RELAX : 0) |
-              ((p.eta_min > 0.0 || p.eta_max < INFINITY) ? F_CLAMP : 0);
+              ((p.eta_min > 0.0 || p.eta_max < INFINITY) ? F_CLAMP : 0) | (p.etasm_spread > 0 ? F_SMBUF : 0);
+    a.spread = p.etasm_spread;
     a.bc_mode = p.bc_mode;
     a.r = p.r;
     a.sh = p.sh;
@@ -230,6 +279,10 @@ void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_
     a.rchunk = pick_rchunk(c, a.b.nr, nx * ny, sizeof(R) == 4);
     const int nz = (a.b.nr + a.rchunk - 1) / a.rchunk;
     if (a.flags & F_CHECK) hipLaunchKernelGGL(k_zero_red, dim3(1), dim3(64), 0, c->stream, a.red);
+    // spread etaSM of the old viscosity into SM (DT_R as the temporary: the
+    // main kernel writes DT_R only in a last iteration, over every cell); a
+    // correction sweep's caller fills its SM field once per outer step
+    if ((a.flags & F_SMBUF) && !(a.flags & F_CORR)) launch_etasm_spread_box<R>(c, a.b, a.eta, a.SM, a.DT, a.spread);
     const int T = nx * ny * nz;
     const int grid = ((T + 7) / 8) * 8;
     if (ev_main0) (void)hipEventRecord(ev_main0, c->stream);
@@ -243,13 +296,17 @@ void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_
         SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15)
         SSPT_FL(32) SSPT_FL(33) SSPT_FL(35) SSPT_FL(36) SSPT_FL(37) SSPT_FL(39)
         SSPT_FL(40) SSPT_FL(41) SSPT_FL(43) SSPT_FL(44) SSPT_FL(45) SSPT_FL(47)
+        SSPT_FL(64) SSPT_FL(65) SSPT_FL(67) SSPT_FL(68) SSPT_FL(69) SSPT_FL(71)
+        SSPT_FL(72) SSPT_FL(73) SSPT_FL(75) SSPT_FL(76) SSPT_FL(77) SSPT_FL(79)
+        SSPT_FL(96) SSPT_FL(97) SSPT_FL(99) SSPT_FL(100) SSPT_FL(101) SSPT_FL(103)
+        SSPT_FL(104) SSPT_FL(105) SSPT_FL(107) SSPT_FL(108) SSPT_FL(109) SSPT_FL(111)
 #undef SSPT_FL
         default: break; // CHECK without LAST is never requested in FP64
         }
     } else {
         switch (am.flags) {
 #define SSPT_FL(v) case v: hipLaunchKernelGGL((k_fused_main<R, v>), G, B, 0, c->stream, am, nx, ny, nz); break;
-        SSPT_FL(16) SSPT_FL(18) SSPT_FL(48) SSPT_FL(50)
+        SSPT_FL(16) SSPT_FL(18) SSPT_FL(48) SSPT_FL(50) SSPT_FL(80) SSPT_FL(82) SSPT_FL(112) SSPT_FL(114)
 #undef SSPT_FL
         default: break; // FP32 runs correction sweeps only
         }

@@ -55,7 +55,10 @@ constexpr double PI_ = 3.14159265358979323846;
 // power law; homogeneous BCs come from sh = 0).
 // F_PAIRED: the in and out buffers of each state field are the two halves of
 // one allocation (one field base serves both, see fat_pp / wout).
-enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP = 8, F_CORR = 16, F_PAIRED = 32 };
+// F_SMBUF: etaSM is read from the SM field (filled before the launch, with
+// the copy_BC'd values at every cell) instead of being computed from the
+// old eta plane by plane; the wide (radius-2) etaSM option uses it.
+enum Flags { F_LAST = 1, F_CHECK = 2, F_RELAX = 4, F_CLAMP = 8, F_CORR = 16, F_PAIRED = 32, F_SMBUF = 64 };
 
 struct BoxG {
     int nr, nt, np;
@@ -154,6 +157,7 @@ template <typename R> struct Args {
     // pure shear: prescribed Vth at the th = th[0] and th = th[nt-1] boundaries
     // (pure_shear_vth, evaluated on the host)
     R vth0, vth1;
+    int spread; // etaSM spread (F_SMBUF; filled before the launch)
     int rchunk; // r-steps per block along grid z
     // byte offsets of the in and out halves of the state fields (viscosity
     // apart: it is frozen in a correction sweep); 0 unless F_PAIRED, where
@@ -328,6 +332,7 @@ template <typename R> struct Thread {
     R e0, e1, e2;                       // old eta at s, s+1, s+2
     R nvr, nvt, nvp, ne, ringJ, ringK;  // prefetched: V(s+2), eta(s+3), eta ring of plane s+2
     R sm0, sm1;                         // etaSM at s, s+1
+    R nsm;                              // F_SMBUF: prefetched etaSM of the next step's plane
     R cRR, cTT, cPP, cDT, crho;         // sigma/DT/rho of plane s-1 (VR face s)
     R rt0, rt0n, rp0, rp0n;             // new tau_Rth/tau_Rph of face s (own, +1 neighbour)
     R pG_rt, pE_rt, pG_rp, pE_rp;       // partial 4-point sums of plane s for face s+1
@@ -429,6 +434,7 @@ template <typename R, int FL> SSPT_DH inline void thread_prologue(Thread<R> &T, 
     T.nvt = fat_in(a.VT, a.pp_in, b.o32(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc));
     T.nvp = fat_in(a.VP, a.pp_in, b.o32(clampi(start + 1, 0, b.nr - 1), T.jc, T.kc));
     T.ringJ = T.ringK = R(0);
+    T.nsm = (FL & F_SMBUF) ? fat(a.SM, b.o32(clampi(start + 1, 0, b.nr - 1), jcc, kcc)) : R(1);
     load_ring(T, a, start + 1);
     load_centre_old<R, FL>(T, a, start);
     load_face_old(T, a, start);
@@ -480,6 +486,53 @@ template <typename R> SSPT_DH inline R star(const Lds<R> &L, int ty, int tx, R e
                      std::fmax(L.ETA[ty + 1][tx], L.ETA[ty + 1][tx + 2]));
 }
 
+// maxloc! followed by copy_BC! at cell (i, j, k) of a centre field src: the
+// 7-point star maximum at the cell clamped into the interior, with the
+// nesting of the star above.  Run over every cell it fills the SM field that
+// F_SMBUF reads.
+template <typename R> SSPT_DH inline R star_bc_at(const BoxG &b, const R *src, int i, int j, int k)
+{
+    const int ci = clampi(i, 1, b.nr - 2), cj = clampi(j, 1, b.nt - 2), ck = clampi(k, 1, b.np - 2);
+    const long long o = b.off(ci, cj, ck);
+    return std::fmax(std::fmax(std::fmax(std::fmax(src[o - b.si], src[o + b.si]), src[o]),
+                               std::fmax(src[o - b.sj], src[o + b.sj])),
+                     std::fmax(src[o - 1], src[o + 1]));
+}
+
+// Spread etaSM (the build option ss_params.etasm_spread = m): one of three
+// separable passes (axis 0 = r, 1 = th, 2 = phi) of
+//     out(i) = max over |a| <= m of src(clamp(i + a e_axis)) * w[|a|],
+// w[0] = 1, w[a] = w[a-1] * 0.1.  After the three passes a cell's etaSM is
+// at least a tenth of its neighbour's within m cells of a stiff cell, so
+// the pseudo-time steps of adjacent cells never differ by more than that
+// factor, whatever the viscosity jump.
+constexpr int SPREAD_MAX = 8;
+constexpr double SPREAD_DECAY = 0.1;
+template <typename R> struct SpreadW {
+    R w[SPREAD_MAX + 1];
+};
+template <typename R> SSPT_DH inline SpreadW<R> spread_weights()
+{
+    SpreadW<R> s;
+    s.w[0] = R(1);
+    for (int a = 1; a <= SPREAD_MAX; ++a) s.w[a] = s.w[a - 1] * R(SPREAD_DECAY);
+    return s;
+}
+template <typename R>
+SSPT_DH inline R spread_at(const BoxG &b, const R *src, int i, int j, int k, int axis, int m, const SpreadW<R> &w)
+{
+    const int n = axis == 0 ? b.nr : (axis == 1 ? b.nt : b.np);
+    const int c = axis == 0 ? i : (axis == 1 ? j : k);
+    const long long st = axis == 0 ? b.si : (axis == 1 ? b.sj : 1);
+    const long long o = b.off(i, j, k) - (long long)c * st;   // the line's start
+    R v = src[o + (long long)c * st];
+    for (int a = 1; a <= m; ++a) {
+        v = std::fmax(v, src[o + (long long)clampi(c - a, 0, n - 1) * st] * w.w[a]);
+        v = std::fmax(v, src[o + (long long)clampi(c + a, 0, n - 1) * st] * w.w[a]);
+    }
+    return v;
+}
+
 // ---- phase 2: etaSM(s+1); C(s) = P, tau_nn, DT at centres; tau_thph(s); VR face s
 template <typename R, int FL>
 SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, bool wr)
@@ -489,7 +542,10 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
     const int nr = b.nr, nt = b.nt, np = b.np;
     const int p = s + 1;
     // etaSM of plane p after copy_BC!: star at the clamped position
-    if (p >= 1 && p <= nr - 2) {
+    if (FL & F_SMBUF) {
+        T.sm1 = T.nsm;
+        T.nsm = fat(a.SM, b.o32(clampi(p + 1, 0, nr - 1), clampi(T.jc, 0, nt - 1), clampi(T.kc, 0, np - 1)));
+    } else if (p >= 1 && p <= nr - 2) {
         const int cj = clampi(T.j, 1, nt - 2), ck = clampi(T.k, 1, np - 2);
         if (cj == T.j && ck == T.k) {
             T.sm1 = star(L, T.ty, T.tx, T.e0, T.e2);
@@ -502,7 +558,9 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
     } else if (p == nr - 1) {
         T.sm1 = T.sm0;
     }
-    if (s == 0) T.sm0 = T.sm1; // etaSM(0) = etaSM(1)
+    // etaSM(0) = etaSM(1) after copy_BC!; a filled SM field (F_SMBUF) holds
+    // plane 0's own values, which sm0 already has
+    if (!(FL & F_SMBUF) && s == 0) T.sm0 = T.sm1;
     L.SM[T.ty][T.tx] = T.sm1;
 
     const bool centre = T.j >= 0 && T.j < nt && T.k >= 0 && T.k < np;
@@ -540,7 +598,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             st_out<R, FL>(a.oPP, a.tPP, o, a.pp_out, T.nPP);
             if ((FL & F_LAST) && !(FL & F_CORR)) {
                 fat(a.DT, o) = T.nDT;
-                fat(a.SM, o) = T.sm0;
+                if (!(FL & F_SMBUF)) fat(a.SM, o) = T.sm0;   // F_SMBUF: the SM field already holds it
             }
         }
     } else {
@@ -626,7 +684,9 @@ SSPT_DH inline void phase3(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
         T.pE_rp2 = eK + T.e1;
         T.gtp2 = (((gs * smJK + gs * smJ) + gs * smK) + gs * T.sm1) * R(0.25);
         T.etp2 = (((eJK + eJ) + eK) + T.e1) * R(0.25);
-        if (s == 0) { // etaSM(0,:,:) = etaSM(1,:,:): plane-0 sums of gs*etaSM are those of plane 1
+        // etaSM(0,:,:) = etaSM(1,:,:) after copy_BC!: plane-0 sums of gs*etaSM
+        // are those of plane 1 (with F_SMBUF plane 0 was read as it is)
+        if (!(FL & F_SMBUF) && s == 0) {
             T.pG_rt = T.pG_rt2;
             T.pG_rp = T.pG_rp2;
         }

@@ -70,7 +70,9 @@ struct FieldSet {
 // FP32 boxes of the defect-correction solve: two correction sets
 // (VR, Vth, Vph, P, tau x6 at 0..9 and 10..19), the frozen viscosity and the
 // forcing (divV and the momentum residual of the FP64 iterate).
-enum { SS_F32_ETA = 20, SS_F32_GP = 21, SS_F32_FVR = 22, SS_F32_FVT = 23, SS_F32_FVP = 24, SS_F32_NBOX = 25 };
+// (SM, TMP: the radius-2 etaSM of the frozen viscosity and its temporary)
+enum { SS_F32_ETA = 20, SS_F32_GP = 21, SS_F32_FVR = 22, SS_F32_FVT = 23, SS_F32_FVP = 24, SS_F32_SM = 25, SS_F32_TMP = 26,
+       SS_F32_NBOX = 27 };
 
 // FP64 boxes of the adjoint solve (sspt_adjoint.hip lists them)
 enum { SS_ADJ_NBOX = 50 };

@@ -6,6 +6,11 @@ namespace sspt {
 
 void launch_equal3(ss_ctx *c, const FieldSet &s);
 void launch_maxloc(ss_ctx *c, double *SM, const double *eta);
+// Spread etaSM in place (ss_params.etasm_spread = m, 1 <= m <= SS_SPREAD_MAX):
+// three separable passes r, th, phi of max_{|a|<=m} SM(clamp(i+a)) * 0.1^|a|,
+// with tmp (a box of the same layout) as the intermediate.
+constexpr int SS_SPREAD_MAX = 8;
+void launch_etasm_spread(ss_ctx *c, double *SM, double *tmp, int m);
 void launch_copy_bc(ss_ctx *c, double *A, int n1, int n2, int n3);
 void launch_compute_P(ss_ctx *c, const FieldSet &s, double _dr, double _dth, double _dph, double dmp, double vpdt,
                       double lr);

@@ -247,6 +247,7 @@ void launch_residual(ss_ctx *c, const FieldSet &s, double *const tau[6], float *
 }
 
 template <typename R> void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_main1);
+template <typename R> void launch_etasm_spread_box(ss_ctx *c, const fused::BoxG &b, const R *eta, R *sm, R *tmp, int m);
 
 } // namespace sspt
 
@@ -400,6 +401,8 @@ int ss_pt_solve_mixed(ss_ctx *c, const ss_params *p, const ss_mixed_opts *o, ss_
     if (!c->grid_set) return mfail(c, SS_ERR_STATE, "ss_set_grid was not called");
     if (o->max_outer < 1 || o->max_outer > SS_MIXED_MAX_OUTER || o->max_inner < 1 || o->nout_inner < 1)
         return mfail(c, SS_ERR_ARG, "bad mixed-solve options");
+    if (p->etasm_spread < 0 || p->etasm_spread > SS_SPREAD_MAX)
+        return mfail(c, SS_ERR_ARG, "etasm_spread must be in [0, 8]");
     if (hipSetDevice(c->device) != hipSuccess) return mfail(c, SS_ERR_STATE, "hipSetDevice");
     int rc = ensure_f32(c);
     if (rc != SS_OK) return rc;
@@ -458,6 +461,10 @@ int ss_pt_solve_mixed(ss_ctx *c, const ss_params *p, const ss_mixed_opts *o, ss_
         // frozen FP32 viscosity; zero correction (both buffer sets)
         hipLaunchKernelGGL(k_to_f32, mgrid(b.nr, b.nt, b.np), MB, 0, c->stream, b, F[SS_F32_ETA], c->cur.f[SS_ETA], b.nr,
                            b.nt, b.np);
+        // spread etaSM of the frozen viscosity, once for all sweeps of the step
+        const bool wide = p->etasm_spread > 0;
+        if (wide) launch_etasm_spread_box<float>(c, a.b, F[SS_F32_ETA], F[SS_F32_SM], F[SS_F32_TMP], p->etasm_spread);
+        a.SM = wide ? F[SS_F32_SM] : nullptr;
         // the first sweep reads set A (= 0) and rewrites every point of set B
         for (int q = 0; q < 10; ++q)
             hipLaunchKernelGGL(k_zero_f32, dim3(2048), dim3(256), 0, c->stream, F[q], c->box_elems);
@@ -471,7 +478,7 @@ int ss_pt_solve_mixed(ss_ctx *c, const ss_params *p, const ss_mixed_opts *o, ss_
             a.oVR = out[0]; a.oVT = out[1]; a.oVP = out[2]; a.oP = out[3];
             a.oRR = out[4]; a.oTT = out[5]; a.oPP = out[6]; a.oRT = out[7]; a.oRP = out[8]; a.oTP = out[9];
             const bool check = it % o->nout_inner == 0;
-            a.flags = fused::F_CORR | (check ? fused::F_CHECK : 0);
+            a.flags = fused::F_CORR | (check ? fused::F_CHECK : 0) | (wide ? fused::F_SMBUF : 0);
             launch_fused_args<float>(c, a, nullptr, nullptr);
             cur = 1 - cur;
             if (check) {

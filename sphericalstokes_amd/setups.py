@@ -46,6 +46,9 @@ class Params:
     eta_max: float = math.inf
     bc_mode: int = BC_SHEAR
     err_drop_nan_vph: int = 0
+    # 0 = the reference's maxloc!; m > 0 = etaSM spread over m cells with a
+    # decay of 10 per cell (build option for sharp viscosity contrasts above ~1e3)
+    etasm_spread: int = 0
 
 
 @dataclass
@@ -163,16 +166,18 @@ def lognormal_log10(shape, *, seed=0, std=0.5, sigma=8.0, clip=2.0) -> np.ndarra
 
 
 def diapir_lognormal(n, *, n_exp=3.0, seed=0, std=0.5, sigma=8.0, clip=2.0, eta_in=1e-3,
-                     bc_mode=BC_FREE_SLIP, cfl_factor=1.0) -> Setup:
+                     bc_mode=BC_FREE_SLIP, cfl_factor=1.0, etasm_spread=5) -> Setup:
     """diapir_sector (sphere of radius 0.1 at (1.5, pi/2, 0), drho = -1,
     power law n_exp) with a random matrix viscosity: log10 eta_matrix from
     lognormal_log10 (seed 0: N(0, 0.5^2), smoothed over 8 cells, contrast
     clipped to 1e4), inclusion viscosity eta_in.  vpdt follows the total
     viscosity contrast as in the reference (CFL = 1/(2 + 4.5 log10 vr)),
-    times cfl_factor: with the full contrast (1e5 with the inclusion) the
-    reference's rule keeps the PT iteration stable up to about 63^3; at
-    127^3 it diverges within 500 iterations (also for n_exp = 1) and needs
-    a factor of about 0.7."""
+    times cfl_factor.  The inclusion's edge is a sharp jump against the
+    matrix next to it (1.2e3 at most at 63^3, 1.4e4 at 95^3, 6.9e3 at 127^3:
+    finer grids resolve more of the random field's high values): with the
+    reference's etaSM any sharp jump above ~3e3 diverges within a hundred
+    iterations, also at half the step, so this setup spreads etaSM over 5
+    cells (etasm_spread = 5) by default."""
     s = diapir_sector(n, n_exp=n_exp, bc_mode=bc_mode)
     f = s.fields
     inc = f["rho"] != 0.0
@@ -184,6 +189,7 @@ def diapir_lognormal(n, *, n_exp=3.0, seed=0, std=0.5, sigma=8.0, clip=2.0, eta_
     vr = float(eta.max() / eta.min())
     CFL = cfl_factor / (2.0 + 4.5 * math.log10(vr))
     s.params.vpdt = (1.0 / (s.nr - 1)) * CFL
+    s.params.etasm_spread = etasm_spread
     s.name = "diapir_lognormal"
     return s
 

@@ -127,6 +127,11 @@ class Solver:
     def maxloc(self):
         self._chk(self._L.ss_maxloc(self._h), "maxloc")
 
+    def spread_etasm(self, m: int):
+        """The etaSM spread of the etasm_spread = m option (no reference
+        counterpart), applied after maxloc + copy_BC(etaSM)."""
+        self._chk(self._L.ss_spread_etasm(self._h, int(m)), "spread_etasm")
+
     def copy_BC(self, name: str):
         self._chk(self._L.ss_copy_bc(self._h, FIELD_ID[canon(name)]), "copy_BC")
 
@@ -166,6 +171,8 @@ class Solver:
             self.equal3()
         self.maxloc()
         self.copy_BC("etaSM")
+        if getattr(p, "etasm_spread", 0) > 0:
+            self.spread_etasm(p.etasm_spread)
         self.compute_P(p._dr, p._dth, p._dph, p.dmp, p.vpdt, p.lr)
         self.compute_TII()
         self.copy_BC("tII")

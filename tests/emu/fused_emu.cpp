@@ -76,8 +76,30 @@ template <typename R> void run_main(const Args<R> &a, int rchunk, unsigned long 
     SSPT_FL(8) SSPT_FL(9) SSPT_FL(11) SSPT_FL(12) SSPT_FL(13) SSPT_FL(15) SSPT_FL(16) SSPT_FL(18)
     SSPT_FL(32) SSPT_FL(33) SSPT_FL(35) SSPT_FL(36) SSPT_FL(37) SSPT_FL(39)
     SSPT_FL(40) SSPT_FL(41) SSPT_FL(43) SSPT_FL(44) SSPT_FL(45) SSPT_FL(47)
+    SSPT_FL(64) SSPT_FL(65) SSPT_FL(67) SSPT_FL(68) SSPT_FL(69) SSPT_FL(71)
+    SSPT_FL(72) SSPT_FL(73) SSPT_FL(75) SSPT_FL(76) SSPT_FL(77) SSPT_FL(79)
+    SSPT_FL(96) SSPT_FL(97) SSPT_FL(99) SSPT_FL(100) SSPT_FL(101) SSPT_FL(103)
+    SSPT_FL(104) SSPT_FL(105) SSPT_FL(107) SSPT_FL(108) SSPT_FL(109) SSPT_FL(111)
 #undef SSPT_FL
-    default: break;
+    default: std::abort(); // a flag combination without an instantiation
+    }
+}
+
+// The etaSM passes before an F_SMBUF iteration (k_star_bc and k_spread on the
+// GPU, same order of buffers): star eta -> tmp, r tmp -> sm, th sm -> tmp,
+// phi tmp -> sm.
+template <typename R> void run_etasm_spread(const BoxG &b, const R *eta, R *sm, R *tmp, int m)
+{
+    const SpreadW<R> w = spread_weights<R>();
+    for (int i = 0; i < b.nr; ++i)
+        for (int j = 0; j < b.nt; ++j)
+            for (int k = 0; k < b.np; ++k) tmp[b.off(i, j, k)] = star_bc_at(b, eta, i, j, k);
+    for (int axis = 0; axis < 3; ++axis) {
+        const R *src = axis == 1 ? sm : tmp;
+        R *dst = axis == 1 ? tmp : sm;
+        for (int i = 0; i < b.nr; ++i)
+            for (int j = 0; j < b.nt; ++j)
+                for (int k = 0; k < b.np; ++k) dst[b.off(i, j, k)] = spread_at(b, src, i, j, k, axis, m, w);
     }
 }
 
@@ -184,8 +206,10 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
         a.g = p->g; a.tC = p->tau_C; a.ex = 1.0 - p->n_exp; a.relax = p->relax;
         a.emin = p->eta_min; a.emax = p->eta_max;
         a.flags = (last ? F_LAST : 0) | (check ? F_CHECK : 0) | (p->relax != 1.0 ? F_RELAX : 0) |
-                  ((p->eta_min > 0.0 || p->eta_max < INFINITY) ? F_CLAMP : 0);
+                  ((p->eta_min > 0.0 || p->eta_max < INFINITY) ? F_CLAMP : 0) |
+                  (p->etasm_spread > 0 ? F_SMBUF : 0);
         a.bc_mode = p->bc_mode; a.r = p->r; a.sh = p->sh;
+        a.spread = p->etasm_spread;
         a.vth0 = pure_shear_vth(th1[0], p->r, p->sh);
         a.vth1 = pure_shear_vth(th1[nt - 1], p->r, p->sh);
         a.rchunk = rchunk > 0 ? rchunk : nr;
@@ -194,6 +218,7 @@ int emu_fused_iterations(int nr, int nt, int np, int pitch, const double *R1, co
             SSPT_ACCESS_UNREGISTER(tbp.data());
             return -1;
         }
+        if (a.flags & F_SMBUF) run_etasm_spread<double>(a.b, a.eta, a.SM, a.DT, a.spread);
         run_main(am, a.rchunk, red);
         run_boundary(a, red);
         std::swap(in, out);

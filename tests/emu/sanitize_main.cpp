@@ -104,6 +104,7 @@ struct Case {
     int nr, nt, np, bc_mode;
     double n_exp, relax, emin, emax;
     int rchunk, paired;
+    int spread = 0;
 };
 
 double worst_rel(const std::vector<double> &a, const std::vector<double> &b)
@@ -142,6 +143,7 @@ int run_case(const Case &c)
     p.dmp = 4.5; p.vpdt = (lr / (nr - 1)) / (2.0 + 4.5 * 3.0); p.lr = lr;
     p.tau_C = 1.0; p.n_exp = c.n_exp; p.g = 1.0; p.r = 1.5; p.sh = 0.7;
     p.relax = c.relax; p.eta_min = c.emin; p.eta_max = c.emax; p.bc_mode = c.bc_mode;
+    p.etasm_spread = c.spread;
     // oracle
     std::vector<std::vector<double>> O = F;
     orc_state os{};
@@ -152,7 +154,7 @@ int run_case(const Case &c)
     for (int f = 0; f < NF; ++f) *op[f] = O[f].data();
     os.R1 = R.data(); os.th1 = th.data(); os.ph1 = ph.data();
     orc_params pp{p._dr, p._dth, p._dph, p.dmp, p.vpdt, p.lr, p.tau_C, p.n_exp, p.g, p.r, p.sh, p.relax,
-                  p.eta_min, p.eta_max, p.bc_mode, 0};
+                  p.eta_min, p.eta_max, p.bc_mode, 0, c.spread};
     for (int it = 0; it < niter; ++it) orc_pt_step(&os, &pp, 1);
     // emulation on exact-size box allocations
     const int pitch = ((np + 1) + 63) / 64 * 64;
@@ -279,6 +281,9 @@ int main()
         {13, 10, 126, SS_BC_PURE_SHEAR, 3.0, 1.0, 0.0, inf, 0, 1},
         {8, 22, 127, SS_BC_FREE_SLIP, 3.0, 0.5, 0.0, inf, 3, 0},
         {12, 31, 17, SS_BC_SHEAR, 5.0, 1.0, 1e-2, 1e2, 7, 1},
+        // etaSM spread (F_SMBUF): window wider than the short axes
+        {5, 13, 64, SS_BC_FREE_SLIP, 1.0, 1.0, 0.0, inf, 0, 1, 8},
+        {9, 12, 65, SS_BC_PURE_SHEAR, 3.0, 0.5, 1e-4, 1e4, 4, 0, 3},
     };
     int bad = 0;
     for (const Case &c : cases) {

@@ -38,6 +38,7 @@ def test_create_rejects_bad_sizes_without_gpu():
 
 
 def test_params_struct_layout():
-    # ss_params: 14 doubles + 2 ints
-    assert C.sizeof(_lib.ss_params) == 14 * 8 + 2 * 4
+    # ss_params: 14 doubles + 3 ints, padded to 8 bytes
+    assert C.sizeof(_lib.ss_params) == 14 * 8 + 4 * 4
     assert C.sizeof(_lib.ss_result) == 4 + 4 + 8 + 24 + 8
+
