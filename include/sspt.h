@@ -135,6 +135,19 @@ int ss_bc_ph(ss_ctx *ctx, int bc_mode, double r, double sh);  /* BC_ph!        :
 /* check_err! (:332-344) + the host max-reductions of :428-432.
  * err_comp[3] = max|V_err|/max|V| per component; *err = their max. */
 int ss_check_err(ss_ctx *ctx, int drop_nan_vph, double err_comp[3], double *err);
+/* ss_check_err without a context and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight solver or serving loops).  VR, Vth, Vph, VR_err,
+ * Vth_err, Vph_err are caller-owned device arrays of the reference shapes
+ * (any element order: the operation is element-wise); V_err <- V - V_err,
+ * then out[0..2] = max|V_err|/max|V| per component and out[3] = err, as
+ * ss_check_err computes them, into the caller's device buffer out[4].
+ * Enqueued on `stream` (a hipStream_t; NULL = the null stream) without
+ * synchronising.  The reduction partials live in a 3 MiB static device
+ * array per GPU: at most 268 435 456 elements per component (SS_ERR_ARG
+ * beyond); calls that overlap on one GPU race on it. */
+int ss_check_err_noalloc(const double *VR, const double *Vth, const double *Vph, double *VR_err, double *Vth_err,
+                         double *Vph_err, int nr, int nth, int nph, int drop_nan_vph, double *out, void *stream);
 
 /* ---- fused PT loop ---- */
 /* Runs `niter` PT iterations with the reference's loop semantics

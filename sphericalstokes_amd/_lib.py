@@ -92,6 +92,7 @@ SIGNATURES = {
     "ss_bc_th": (I, [P, I, D, D]),
     "ss_bc_ph": (I, [P, I, D, D]),
     "ss_check_err": (I, [P, I, DP, DP]),
+    "ss_check_err_noalloc": (I, [P, P, P, P, P, P, I, I, I, I, P, P]),
     "ss_pt_iterate": (I, [P, C.POINTER(ss_params), I]),
     "ss_pt_solve": (I, [P, C.POINTER(ss_params), I, I, D, C.POINTER(ss_result)]),
     "ss_pt_solve_unfused": (I, [P, C.POINTER(ss_params), I, I, D, C.POINTER(ss_result)]),

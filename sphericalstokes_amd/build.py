@@ -25,7 +25,8 @@ ORACLE_DIR = ROOT / "oracle"
 EMU_SRC = [ROOT / "tests" / "emu" / "fused_emu.cpp", ROOT / "tests" / "emu" / "adj_emu.cpp"]
 EMU_LIB = ROOT / "tests" / "emu" / "libsspt_emu.so"
 
-HIP_SOURCES = ["sspt_api.hip", "sspt_kernels.hip", "sspt_fused.hip", "sspt_mixed.hip", "sspt_adjoint.hip"]
+HIP_SOURCES = ["sspt_api.hip", "sspt_kernels.hip", "sspt_fused.hip", "sspt_mixed.hip", "sspt_adjoint.hip",
+               "sspt_noalloc.hip"]
 ARCH = os.environ.get("SSPT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -42,3 +42,14 @@ def test_params_struct_layout():
     assert C.sizeof(_lib.ss_params) == 14 * 8 + 4 * 4
     assert C.sizeof(_lib.ss_result) == 4 + 4 + 8 + 24 + 8
 
+
+def test_check_err_noalloc_rejects_without_gpu():
+    """Argument checks of the allocation-free check run before any HIP call."""
+    L = _lib.lib()
+    dummy = C.c_void_p(8)
+    args = [dummy] * 6
+    assert L.ss_check_err_noalloc(None, *args[1:], 10, 10, 10, 0, dummy, None) == _lib.SS_ERR_ARG
+    assert L.ss_check_err_noalloc(*args, 10, 10, 10, 0, None, None) == _lib.SS_ERR_ARG
+    assert L.ss_check_err_noalloc(*args, 0, 10, 10, 0, dummy, None) == _lib.SS_ERR_ARG
+    # 700^3 cells: beyond the scratch's 268 435 456 elements per component
+    assert L.ss_check_err_noalloc(*args, 700, 700, 700, 0, dummy, None) == _lib.SS_ERR_ARG
