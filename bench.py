@@ -13,10 +13,12 @@ reference CPU path) timed on a bounded sample of the same workload.
 "mixed_precision" holds the second half of the metric: the cost of one FP32
 correction sweep on the same 511^3 workload against one FP64 iteration, and
 the time to reach an FP64 residual of 1e-8 with the plain FP64 solve and
-with the FP32-inner / FP64 defect-correction solve (linear diapir_sector at
---ttt-n^3, default 127^3: about three minutes of GPU time, where the FP32
-sweeps are already bandwidth-bound; 63^3 is latency-bound and understates
-the gain).
+with the FP32-inner / FP64 defect-correction solve on the mixed-precision
+configuration's own field (the power-law diapir with the smoothed log-normal
+matrix viscosity, etaSM spread over 5 cells) at --ttt-n^3, default 127^3:
+about five minutes of GPU time (887000 FP64 iterations), the largest size at
+which the FP64 solve reaches 1e-8 within minutes; --ttt-setup diapir selects
+the linear diapir_sector instead.
 "adjoint" holds the cost of one adjoint iteration on the same workload
 against one FP64 iteration, and the wall times of a forward solve and of
 the adjoint solve that gives the gradient (power-law diapir at --adjoint-n^3).
@@ -140,7 +142,8 @@ def time_to_tolerance(n, tol, tol_inner, setup="diapir"):
 
     if setup == "lognormal":
         st = setups.diapir_lognormal(n)
-        what = f"diapir_lognormal {n}^3 (n_exp=3, log10 eta_matrix ~ N(0, 0.5^2) seed 0, smoothed 8 cells, contrast <= 1e4)"
+        what = (f"diapir_lognormal {n}^3 (n_exp=3, log10 eta_matrix ~ N(0, 0.5^2) seed 0, smoothed 8 cells, "
+                f"contrast <= 1e4, etaSM spread {st.params.etasm_spread})")
     else:
         st = setups.diapir_sector(n, n_exp=1.0, bc_mode=BC_FREE_SLIP)
         what = f"diapir_sector {n}^3, n_exp=1"
@@ -333,7 +336,7 @@ def main():
     ap.add_argument("--mixed-large-n", type=int, default=0,
                     help="also run outer steps of the mixed solve on the log-normal power-law diapir at this "
                          "size (767 for the full configuration); 0 skips it")
-    ap.add_argument("--ttt-setup", choices=["diapir", "lognormal"], default="diapir",
+    ap.add_argument("--ttt-setup", choices=["diapir", "lognormal"], default="lognormal",
                     help="time-to-tolerance workload: the linear diapir_sector, or the power-law diapir "
                          "with the smoothed log-normal matrix viscosity")
     ap.add_argument("--ttt-tol-inner", type=float, default=1e-5,
