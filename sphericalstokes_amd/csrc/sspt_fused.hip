@@ -137,46 +137,123 @@ __global__ void __launch_bounds__(256) k_fused_boundary(Args<R> a)
     if (a.flags & F_CHECK) flush_max(a, mdv, mv, (threadIdx.x + threadIdx.y * blockDim.x) & 63);
 }
 
-// dst = copy_BC!(maxloc!(src)) over every cell (star_bc_at).
-template <typename R> __global__ void __launch_bounds__(256) k_star_bc(BoxG b, const R *__restrict__ src, R *__restrict__ dst)
+// The etaSM spread (spread_at's three separable passes over the cells, after
+// the star of maxloc! + copy_BC!) in two launches.
+//
+// k_spread_r: the r pass, fused with the star when STAR.  A thread owns one
+// (j, k) column of a chunk of rch planes and keeps the 2M+1 source values
+// around its plane in registers (source = star_bc_at(src) or src itself, at
+// the clamped plane), loading one plane ahead per step.
+template <typename R, int M, bool STAR>
+__global__ void __launch_bounds__(256) k_spread_r(BoxG b, const R *__restrict__ src, R *__restrict__ dst, int rch)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k < b.np && j < b.nt && i < b.nr) dst[b.off(i, j, k)] = star_bc_at(b, src, i, j, k);
-}
-
-// One separable pass of the etaSM spread (spread_at) over every cell.
-template <typename R>
-__global__ void __launch_bounds__(256) k_spread(BoxG b, const R *__restrict__ src, R *__restrict__ dst, int axis, int m)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    if (k >= b.np || j >= b.nt) return;
+    const int i0 = blockIdx.z * rch, i1 = min(i0 + rch, b.nr);
     const SpreadW<R> w = spread_weights<R>();
-    if (k < b.np && j < b.nt && i < b.nr) dst[b.off(i, j, k)] = spread_at(b, src, i, j, k, axis, m, w);
+    auto S = [&](int i) -> R {
+        const int ic = clampi(i, 0, b.nr - 1);
+        if constexpr (STAR) return star_bc_at(b, src, ic, j, k);
+        else return src[b.off(ic, j, k)];
+    };
+    R win[2 * M + 1];
+#pragma unroll
+    for (int q = 0; q < 2 * M + 1; ++q) win[q] = S(i0 - M + q);
+#pragma unroll 2
+    for (int i = i0; i < i1; ++i) {
+        const R ahead = S(i + M + 1);
+        R v = win[M];
+#pragma unroll
+        for (int a = 1; a <= M; ++a) {
+            v = std::fmax(v, win[M - a] * w.w[a]);
+            v = std::fmax(v, win[M + a] * w.w[a]);
+        }
+        dst[b.off(i, j, k)] = v;
+#pragma unroll
+        for (int q = 0; q < 2 * M; ++q) win[q] = win[q + 1];
+        win[2 * M] = ahead;
+    }
 }
 
-// With eta != null: sm <- spread(maxloc! + copy_BC! of eta) in four passes
-// (star: eta -> tmp, r: tmp -> sm, th: sm -> tmp, phi: tmp -> sm).  With
-// eta == null: sm <- spread(sm) (r: sm -> tmp, th: tmp -> sm, phi: sm ->
-// tmp, then tmp copied back).
+// k_spread_thph: the th and phi passes of one r-plane tile (SPT_J rows x 64
+// columns of outputs) through LDS: the tile and a halo of M clamped rows
+// and columns of the r-pass result are staged, the th pass runs over the
+// tile's rows and all staged columns, the phi pass over the outputs.  A
+// staged row / column holds the value at the clamped index, so each pass
+// reads src(clamp(x + a)) as spread_at does.
+constexpr int SPT_J = 32;
+template <typename R, int M>
+__global__ void __launch_bounds__(256) k_spread_thph(BoxG b, const R *__restrict__ src, R *__restrict__ dst)
+{
+    constexpr int W = 64 + 2 * M, H = SPT_J + 2 * M;
+    __shared__ R in[H][W];
+    __shared__ R mid[SPT_J][W];
+    const int k0 = blockIdx.x * 64, j0 = blockIdx.y * SPT_J, i = blockIdx.z;
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    const SpreadW<R> w = spread_weights<R>();
+    for (int e = t; e < H * W; e += 256) {
+        const int r = e / W, c = e - r * W;
+        in[r][c] = src[b.off(i, clampi(j0 - M + r, 0, b.nt - 1), clampi(k0 - M + c, 0, b.np - 1))];
+    }
+    __syncthreads();
+    for (int e = t; e < SPT_J * W; e += 256) {
+        const int r = e / W, c = e - r * W;
+        R v = in[r + M][c];
+#pragma unroll
+        for (int a = 1; a <= M; ++a) {
+            v = std::fmax(v, in[r + M - a][c] * w.w[a]);
+            v = std::fmax(v, in[r + M + a][c] * w.w[a]);
+        }
+        mid[r][c] = v;
+    }
+    __syncthreads();
+    for (int e = t; e < SPT_J * 64; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        const int j = j0 + r, k = k0 + c;
+        if (j >= b.nt || k >= b.np) continue;
+        R v = mid[r][c + M];
+#pragma unroll
+        for (int a = 1; a <= M; ++a) {
+            v = std::fmax(v, mid[r][c + M - a] * w.w[a]);
+            v = std::fmax(v, mid[r][c + M + a] * w.w[a]);
+        }
+        dst[b.off(i, j, k)] = v;
+    }
+}
+
+// r-chunk of k_spread_r: enough chunks to fill the device at small sizes,
+// few enough that the 2M warm-up planes per chunk stay a small share
+static int spread_rchunk(int nr) { return std::max(8, std::min(32, nr / 8)); }
+
+template <typename R, int M> static void spread_two_pass(ss_ctx *c, const BoxG &b, const R *eta, R *sm, R *tmp)
+{
+    const int rch = spread_rchunk(b.nr);
+    const dim3 Gr((b.np + 63) / 64, (b.nt + 3) / 4, (b.nr + rch - 1) / rch), Br(64, 4);
+    if (eta) hipLaunchKernelGGL((k_spread_r<R, M, true>), Gr, Br, 0, c->stream, b, eta, tmp, rch);
+    else hipLaunchKernelGGL((k_spread_r<R, M, false>), Gr, Br, 0, c->stream, b, (const R *)sm, tmp, rch);
+    const dim3 Gt((b.np + 63) / 64, (b.nt + SPT_J - 1) / SPT_J, b.nr), Bt(64, 4);
+    hipLaunchKernelGGL((k_spread_thph<R, M>), Gt, Bt, 0, c->stream, b, (const R *)tmp, sm);
+}
+
+// sm <- spread(maxloc! + copy_BC! of eta) with eta != null, sm <- spread(sm)
+// with eta == null; tmp (a box of the same shape) holds the r-pass result.
+// m is checked against SPREAD_MAX by every entry point that accepts it.
 template <typename R> void launch_etasm_spread_box(ss_ctx *c, const BoxG &b, const R *eta, R *sm, R *tmp, int m)
 {
-    const dim3 G((b.np + 63) / 64, (b.nt + 3) / 4, b.nr), B(64, 4);
-    R *x = sm, *y = tmp;
-    if (eta) {
-        hipLaunchKernelGGL((k_star_bc<R>), G, B, 0, c->stream, b, eta, tmp);
-        x = tmp;
-        y = sm;
+    switch (m) {
+    case 1: spread_two_pass<R, 1>(c, b, eta, sm, tmp); break;
+    case 2: spread_two_pass<R, 2>(c, b, eta, sm, tmp); break;
+    case 3: spread_two_pass<R, 3>(c, b, eta, sm, tmp); break;
+    case 4: spread_two_pass<R, 4>(c, b, eta, sm, tmp); break;
+    case 5: spread_two_pass<R, 5>(c, b, eta, sm, tmp); break;
+    case 6: spread_two_pass<R, 6>(c, b, eta, sm, tmp); break;
+    case 7: spread_two_pass<R, 7>(c, b, eta, sm, tmp); break;
+    case 8: spread_two_pass<R, 8>(c, b, eta, sm, tmp); break;
+    default: break;
     }
-    hipLaunchKernelGGL((k_spread<R>), G, B, 0, c->stream, b, (const R *)x, y, 0, m);
-    hipLaunchKernelGGL((k_spread<R>), G, B, 0, c->stream, b, (const R *)y, x, 1, m);
-    hipLaunchKernelGGL((k_spread<R>), G, B, 0, c->stream, b, (const R *)x, y, 2, m);
-    if (!eta)
-        (void)hipMemcpyAsync(sm, tmp, (size_t)(b.nr + 1) * (size_t)b.si * sizeof(R), hipMemcpyDeviceToDevice,
-                             c->stream);
 }
+static_assert(SPREAD_MAX == 8, "launch_etasm_spread_box instantiates m = 1..8");
 template void launch_etasm_spread_box<float>(ss_ctx *, const BoxG &, const float *, float *, float *, int);
 
 void launch_etasm_spread(ss_ctx *c, double *SM, double *tmp, int m)

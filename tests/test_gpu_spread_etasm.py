@@ -39,6 +39,28 @@ def _case(shape, seed, spread=3, **kw):
     return st
 
 
+@pytest.mark.parametrize("shape", [(9, 7, 70), (40, 33, 5), (17, 70, 130)])
+def test_spread_pass_every_m_bitwise(shape):
+    """ss_spread_etasm (the r pass, then the th and phi passes through LDS
+    tiles) against orc_spread_etasm for every m, bit for bit, on shapes with
+    dimensions below 2m (clamping at both ends of a line) and above one
+    tile (64 columns, 32 rows)."""
+    from oracle.binding import _ptr, lib
+    st = setups.diapir_sector(shape[0], nth=shape[1], nph=shape[2], n_exp=1.0)
+    rng = np.random.default_rng(sum(shape))
+    with _solver(st) as s:
+        for m in range(1, 9):
+            A = np.asfortranarray(np.exp(3 * rng.standard_normal(shape)))
+            s.upload("etaSM", A)
+            s.spread_etasm(m)
+            got = s.download("etaSM")
+            exp = A.copy(order="F")
+            tmp = np.zeros_like(exp)
+            lib().orc_spread_etasm(_ptr(exp), _ptr(tmp), *shape, m)
+            say("spread", shape, "m", m, "differing", int(np.count_nonzero(got != exp)))
+            assert np.array_equal(got, exp), m
+
+
 @pytest.mark.parametrize("fused", [True, False])
 def test_spread_etasm_bitwise_linear(fused):
     st = _case((21, 40, 70), 11, n_exp=1.0, bc_mode=BC_FREE_SLIP)
