@@ -1,0 +1,257 @@
+#!/usr/bin/env python
+"""Golden vectors from the reference's own kernel text.
+
+Reads the reference scripts under $SSPT_REFERENCE (default /root/reference)
+as text, evaluates their ParallelStencil kernels with the Julia-subset
+evaluator of jl_subset.py (C-library math), and writes inputs and outputs
+to tests/golden/reference_text_vectors.npz (+ .json with the scalars).
+Run it in a container that holds the reference; the tests read only the
+committed vectors.
+
+Cases (grid 7 x 6 x 8 cells, wide sector r in [1, 2], th in [pi/3, 2pi/3],
+phi in [-pi/6, pi/6]; seeded random state in which every kernel changes its
+outputs):
+  * diapir: every @parallel_indices kernel of
+    scripts_diapir/SphericalStokes_diapir.jl (compute_P! with its pseudo-time
+    steps KBDT, GSDT, DT_R; compute_TII!; power_law! with n_exp = 3;
+    compute_V!; the simple-shear BC_R!/BC_th!/BC_ph!; copy_BC!; equal3!;
+    check_err!) one at a time on the same input, and three iterations of
+    the script's loop body (:413-425) with maxloc! as ParallelStencil's
+    @maxloc (the expansion written out in
+    scripts_perf/SphericalStokes_mxpu.jl:295-297);
+  * plateau: the free-slip BC_R!/BC_th!/BC_ph! of
+    scripts_plateau/SphericalStokes_plateau.jl and three iterations of its
+    loop body (:526-538) with its own compute kernels;
+  * relax: power_law! with relaxation 0.5 of
+    scripts_diapir/CartesianStokes_diapir.jl:210-220;
+  * maxloc: the etaSM part of compute_V_etaSM! (SphericalStokes_mxpu.jl);
+  * init: initialize_inclusion! / initialize_velocity! (diapir, 15^3 sector
+    grid, squared-distance test with radius 0.01) and initialize_geometry!
+    (plateau, on the script's scaled 70 x 5 x 5 grid and on 31 x 19 x 23).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(HERE))
+sys.path.insert(0, str(ROOT))
+
+import jl_subset as J  # noqa: E402
+
+REF = Path(os.environ.get("SSPT_REFERENCE", "/root/reference"))
+OUT = HERE / "reference_text_vectors"
+
+SHAPES = {  # reference array shapes, SphericalStokes_diapir.jl:381-399
+    "P": (0, 0, 0), "DT_R": (0, 0, 0), "τ_RR": (0, 0, 0), "τ_θθ": (0, 0, 0), "τ_φφ": (0, 0, 0),
+    "τ_Rθ": (-1, -1, -2), "τ_Rφ": (-1, -2, -1), "τ_θφ": (-2, -1, -1),
+    "VR": (1, 0, 0), "Vθ": (0, 1, 0), "Vφ": (0, 0, 1), "VR_err": (1, 0, 0), "Vθ_err": (0, 1, 0),
+    "Vφ_err": (0, 0, 1), "η": (0, 0, 0), "η_ini": (0, 0, 0), "ηSM": (0, 0, 0), "τII": (0, 0, 0),
+    "ρ": (0, 0, 0),
+}
+# Julia names -> field names of the package (sphericalstokes_amd.fields)
+FIELD = {"P": "P", "DT_R": "DT_R", "τ_RR": "tau_RR", "τ_θθ": "tau_thth", "τ_φφ": "tau_phph",
+         "τ_Rθ": "tau_Rth", "τ_Rφ": "tau_Rph", "τ_θφ": "tau_thph", "VR": "VR", "Vθ": "Vth", "Vφ": "Vph",
+         "VR_err": "VR_err", "Vθ_err": "Vth_err", "Vφ_err": "Vph_err", "η": "eta", "η_ini": "eta_ini",
+         "ηSM": "etaSM", "τII": "tII", "ρ": "rho"}
+
+
+def sector(nr, nth, nph):
+    r, lr, lth, lph = 1.5, 1.0, math.pi / 3.0, math.pi / 3.0
+    R1 = np.linspace(r - lr / 2.0, r + lr / 2.0, nr)
+    th1 = np.linspace(math.pi / 2.0 - lth / 2.0, math.pi / 2.0 + lth / 2.0, nth)
+    ph1 = np.linspace(-lph / 2.0, lph / 2.0, nph)
+    vr = 1e3
+    CFL = 1.0 / (2.0 + 4.5 * math.log10(vr))
+    sc = dict(_dr=(nr - 1) / lr, _dθ=(nth - 1) / lth, _dφ=(nph - 1) / lph, dmp=4.5, vpdt=lr / (nr - 1) * CFL,
+              lr=lr, τ_C=1.0, n_exp=3.0, g=1.0, r=r, sh=1.0)
+    return R1, th1, ph1, sc
+
+
+def grid3(R1, th1, ph1):
+    nr, nth, nph = len(R1), len(th1), len(ph1)
+    R = np.asfortranarray(np.broadcast_to(R1[:, None, None], (nr, nth, nph)).copy())
+    th = np.asfortranarray(np.broadcast_to(th1[None, :, None], (nr, nth, nph)).copy())
+    ph = np.asfortranarray(np.broadcast_to(ph1[None, None, :], (nr, nth, nph)).copy())
+    return R, th, ph
+
+
+def random_state(nr, nth, nph, seed):
+    """Every field non-trivial: V, P, tau, rho ~ N(0, 1); eta, eta_ini,
+    etaSM, DT_R, tII positive log-normal."""
+    g = np.random.default_rng(seed)
+    st = {}
+    for name, d in SHAPES.items():
+        shp = (nr + d[0], nth + d[1], nph + d[2])
+        if name in ("η", "η_ini", "ηSM", "DT_R", "τII"):
+            a = 10.0 ** g.normal(0.0, 0.5, shp)
+        else:
+            a = g.normal(0.0, 1.0, shp)
+        st[name] = np.asfortranarray(a)
+    return st
+
+
+def copy(st):
+    return {k: np.array(v, order="F", copy=True) for k, v in st.items()}
+
+
+class Runner:
+    """`@parallel name(args...)` with arguments looked up by the kernel's
+    own parameter names (state arrays, coordinates, scalars)."""
+
+    def __init__(self, script, st, coords, sc):
+        self.S, self.st, self.coords, self.sc = script, st, coords, sc
+
+    def __call__(self, name, **over):
+        k = self.S.kernels[name]
+        env = {**self.sc, **self.coords, **self.st, **over}
+        self.S.run(name, *[env[p] for p in k.params])
+
+    def copy_BC(self, A):
+        # copy_BC!: SphericalStokes_diapir.jl:120-124
+        for nm in ("copy_bc_r!", "copy_bc_θ!", "copy_bc_φ!"):
+            self(nm, A=A)
+
+
+def maxloc(mx, st, coords, sc):
+    """maxloc!(etaSM, eta): the etaSM statement of compute_V_etaSM!
+    (SphericalStokes_mxpu.jl:295-297), run on scratch velocities."""
+    scratch = copy(st)
+    env = {**sc, **coords, **scratch, "ηSM": st["ηSM"]}
+    k = mx.kernels["compute_V_ηSM!"]
+    mx.run("compute_V_ηSM!", *[env[p] for p in k.params])
+
+
+def loop_body(S, mx, st, coords, sc, bc_args):
+    """One iteration of the scripts' loop body (diapir :413-425, plateau
+    :526-538): equal3!, maxloc! + copy_BC!(etaSM), compute_P!, compute_TII!
+    + copy_BC!(tII), power_law!, compute_V!, BC_R!, BC_th!, BC_ph!."""
+    run = Runner(S, st, coords, sc)
+    run("equal3!")
+    maxloc(mx, st, coords, sc)
+    run.copy_BC(st["ηSM"])
+    run("compute_P!")
+    run("compute_TII!")
+    run.copy_BC(st["τII"])
+    run("power_law!")
+    run("compute_V!")
+    run("BC_R!")
+    run("BC_θ!", **bc_args)
+    run("BC_φ!", **bc_args)
+
+
+def main():
+    diapir = J.Script((REF / "scripts_diapir" / "SphericalStokes_diapir.jl").read_text())
+    plateau = J.Script((REF / "scripts_plateau" / "SphericalStokes_plateau.jl").read_text())
+    cart = J.Script((REF / "scripts_diapir" / "CartesianStokes_diapir.jl").read_text())
+    mx = J.Script((REF / "scripts_perf" / "SphericalStokes_mxpu.jl").read_text())
+
+    arrays, meta = {}, {}
+
+    def put(case, tag, name, a):
+        case = case.replace("θ", "th").replace("φ", "ph")
+        arrays[f"{case}__{tag}__{FIELD.get(name, name)}"] = np.asarray(a)
+
+    nr, nth, nph = 7, 6, 8
+    R1, th1, ph1, sc = sector(nr, nth, nph)
+    R, th, ph = grid3(R1, th1, ph1)
+    coords = {"R": R, "θ": th, "φ": ph}
+
+    # ---- diapir: kernels one at a time, then three loop iterations
+    st0 = random_state(nr, nth, nph, seed=11)
+    meta["grid"] = [nr, nth, nph]
+    meta["scalars"] = {{"_dθ": "_dth", "_dφ": "_dph", "τ_C": "tau_C"}.get(k, k): v for k, v in sc.items()}
+    arrays["R1"], arrays["th1"], arrays["ph1"] = R1, th1, ph1
+    for n, a in st0.items():
+        put("in", "state", n, a)
+    per_kernel = {
+        "compute_P!": ["P", "DT_R", "τ_RR", "τ_θθ", "τ_φφ", "τ_Rθ", "τ_Rφ", "τ_θφ"],
+        "compute_TII!": ["τII"], "power_law!": ["η"], "compute_V!": ["VR", "Vθ", "Vφ"],
+        "BC_R!": ["VR", "Vθ", "Vφ"], "BC_θ!": ["VR", "Vθ", "Vφ"], "BC_φ!": ["VR", "Vθ", "Vφ"],
+        "equal3!": ["VR_err", "Vθ_err", "Vφ_err"], "check_err!": ["VR_err", "Vθ_err", "Vφ_err"],
+    }
+    for kname, outs in per_kernel.items():
+        st = copy(st0)
+        Runner(diapir, st, coords, sc)(kname)
+        for n in outs:
+            assert not np.array_equal(st[n], st0[n]), (kname, n)
+            put("diapir_" + kname.rstrip("!"), "out", n, st[n])
+    st = copy(st0)
+    Runner(diapir, st, coords, sc).copy_BC(st["τII"])
+    put("diapir_copy_BC_tII", "out", "τII", st["τII"])
+    st = copy(st0)
+    maxloc(mx, st, coords, sc)
+    put("maxloc", "out", "ηSM", st["ηSM"])
+    for case, S, bc_args in (("diapir_loop", diapir, {}), ("plateau_loop", plateau, {})):
+        st = copy(st0)
+        for _ in range(3):
+            loop_body(S, mx, st, coords, sc, bc_args)
+        for n in SHAPES:
+            put(case, "out", n, st[n])
+    for kname in ("BC_R!", "BC_θ!", "BC_φ!"):
+        st = copy(st0)
+        Runner(plateau, st, coords, sc)(kname)
+        for n in ("VR", "Vθ", "Vφ"):
+            put("plateau_" + kname.rstrip("!"), "out", n, st[n])
+
+    # ---- relaxation: CartesianStokes_diapir.jl power_law! (relax 0.5)
+    g = np.random.default_rng(5)
+    shp = (nr, nth, nph)
+    rl = {n: np.asfortranarray(10.0 ** g.normal(0.0, 0.5, shp)) for n in ("τII", "η_INI", "η_PL_OLD")}
+    ascii_ = {"τII": "tII", "η_INI": "eta_ini", "η_PL_OLD": "eta_PL_old", "η_PL": "eta_PL", "η": "eta"}
+    for n, a in rl.items():
+        arrays[f"relax__in__{ascii_[n]}"] = a.copy(order="F")
+    rl["η_PL"] = np.asfortranarray(np.full(shp, np.nan))
+    rl["η"] = np.asfortranarray(np.full(shp, np.nan))
+    rsc = {"τ_C": 1.3, "n_exp": 3.0, "relax": 0.5}
+    k = cart.kernels["power_law!"]
+    cart.run("power_law!", *[{**rl, **rsc}[p] for p in k.params])
+    arrays["relax__out__eta_PL"] = rl["η_PL"]
+    arrays["relax__out__eta"] = rl["η"]
+    meta["relax"] = {"tau_C": rsc["τ_C"], "n_exp": rsc["n_exp"], "relax": rsc["relax"]}
+
+    # ---- setup kernels
+    n = 15
+    R1s, th1s, ph1s, scs = sector(n, n, n)
+    Rs, ths, phs = grid3(R1s, th1s, ph1s)
+    A = np.asfortranarray(np.zeros((n, n, n)))
+    diapir.run("initialize_inclusion!", A, Rs, ths, phs, 1.5, 0.1 ** 2, -1.0)
+    arrays["init__R1"], arrays["init__th1"], arrays["init__ph1"] = R1s, th1s, ph1s
+    arrays["init__out__inclusion"] = A
+    assert 0 < np.count_nonzero(A) < A.size
+    V = np.asfortranarray(np.zeros((n, n, n + 1)))
+    Vth = np.asfortranarray(np.broadcast_to(th1s[None, :, None], (n, n, n)).copy())
+    diapir.run("initialize_velocity!", V, Vth, 0.7, 1.5)
+    arrays["init__out__velocity"] = V
+    meta["init"] = {"r": 1.5, "radius": 0.1 ** 2, "in": -1.0, "fact": 0.7}
+
+    from sphericalstokes_amd import plateau as PL
+    for tag, (gnr, gnt, gnp) in (("geo70", (70, 5, 5)), ("geo31", (31, 19, 23))):
+        su = PL.plateau_reference(gnr, gnt, gnp)
+        sp_ = PL.scale(PL.PlateauParams())
+        Rg, tg, pg = grid3(su.R, su.th, su.ph)
+        gp = dict(r=su.params.r, he=sp_.he, hc=sp_.hc, hr=sp_.hr, hl=sp_.hl, dl=sp_.dl, dh=sp_.dh, dt=sp_.dt)
+        for what, d in (("rho", sp_.rho), ("eta", sp_.eta)):
+            vals = (d["l"], d["c"], d["a"])
+            A = np.asfortranarray(np.full((gnr, gnt, gnp), d["m"]))    # rho_m * ones, eta_m * ones
+            k = plateau.kernels["initialize_geometry!"]
+            env = {"A": A, "R": Rg, "θ": tg, "φ": pg, **gp, "gl": vals[0], "gc": vals[1], "ga": vals[2]}
+            plateau.run("initialize_geometry!", *[env[p] for p in k.params])
+            arrays[f"{tag}__out__{what}"] = A
+        arrays[f"{tag}__R1"], arrays[f"{tag}__th1"], arrays[f"{tag}__ph1"] = su.R, su.th, su.ph
+        meta[tag] = {"grid": [gnr, gnt, gnp], **{k: float(v) for k, v in gp.items()}}
+
+    np.savez_compressed(OUT.with_suffix(".npz"), **arrays)
+    OUT.with_suffix(".json").write_text(json.dumps(meta, indent=1, sort_keys=True) + "\n")
+    print(f"wrote {OUT.with_suffix('.npz')} ({len(arrays)} arrays)")
+
+
+if __name__ == "__main__":
+    main()

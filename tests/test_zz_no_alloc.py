@@ -2,7 +2,8 @@
 ss_check_err (context scratch) on the same data: the three component
 errors, err and the V - V_prev arrays bit for bit, on calls in a row with
 shrinking inputs and a NaN case, through the C ABI.  Device arrays are
-allocated here with the HIP runtime that libsspt.so itself uses."""
+allocated here with the HIP runtime that libsspt.so itself uses (looked up
+through its handle), also when torch's copy of the runtime is mapped too."""
 import ctypes as C
 import sys
 
@@ -24,19 +25,11 @@ class Hip:
     H2D, D2H = 1, 2
 
     def __init__(self):
-        # the HIP runtime that libsspt.so loaded (same file, hence the same instance)
+        # the HIP runtime libsspt.so itself uses: symbols looked up through
+        # libsspt.so's handle resolve in its own dependency tree, whatever
+        # other copy of libamdhip64 (torch's) the process also maps
         from sphericalstokes_amd import _lib
-        _lib.lib()
-        # (both the ROCm installation's and torch's copy have the soname
-        # libamdhip64.so.7, so a process maps exactly one of them: the one
-        # loaded first, which libsspt.so then also uses)
-        paths = set()
-        with open("/proc/self/maps") as fh:
-            for line in fh:
-                if "libamdhip64.so" in line:
-                    paths.add(line.split()[-1])
-        assert len(paths) == 1, f"expected one HIP runtime in the process, found {sorted(paths)}"
-        self.L = C.CDLL(paths.pop())
+        self.L = _lib.lib()
         self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         self.L.hipFree.argtypes = [C.c_void_p]
         self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
