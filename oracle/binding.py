@@ -1,4 +1,5 @@
-"""ctypes binding of the CPU oracle (oracle/libsspt_oracle.so).
+"""ctypes binding of the CPU oracle (oracle/libsspt_oracle.so): the spherical
+iteration, its adjoint and the Cartesian twin.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, as the checker.  The product never uses it.
@@ -219,6 +220,122 @@ class Oracle:
         it = self.L.orc_adj_solve(C.byref(self.s), C.byref(self.p), float(r), float(radius), C.byref(a),
                                   int(maxiter), int(ncheck), float(tol), _ptr(g_rho), _ptr(g_ini))
         return it, arrs, g_rho, g_ini
+
+
+# ---- the Cartesian twin (oracle/sspt_cart_oracle.c)
+_CART = ["P", "tXX", "tYY", "tZZ", "tXY", "tXZ", "tYZ", "VX", "VY", "VZ", "ERR_VX", "ERR_VY", "ERR_VZ", "rho",
+         "eta", "eta_ini", "eta_PL", "eta_PL_old", "etaSM", "DT_R", "tII"]
+# orc_cart member -> field name of sphericalstokes_amd.cartesian
+_CART_NAME = {"tXX": "tau_XX", "tYY": "tau_YY", "tZZ": "tau_ZZ", "tXY": "tau_XY", "tXZ": "tau_XZ", "tYZ": "tau_YZ",
+              "eta_ini": "eta_INI", "eta_PL_old": "eta_PL_OLD"}
+CART_BC = {"bc_x_lin": 0, "bc_x_0": 1, "copy_bc_x": 2, "bc_y_fact": 3, "bc_y_0": 4, "bc_z_0": 5, "copy_bc_z": 6}
+
+
+class orc_cart(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + [(f, DP) for f in _CART] + [("y1", DP)]
+
+
+class orc_cart_params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("_dx", "_dy", "_dz", "dmp", "vpdt", "lx", "tau_C", "n_exp", "relax",
+                                         "g", "sh")]
+
+
+def _cart_lib():
+    L = lib()
+    if not getattr(L, "_cart_sigs", False):
+        SP, PP, I, D = C.POINTER(orc_cart), C.POINTER(orc_cart_params), C.c_int, C.c_double
+        for n, res, args in (("orc_cart_equal", None, [DP, DP, C.c_size_t]),
+                             ("orc_cart_maxloc", None, [SP]),
+                             ("orc_cart_copy_boundary", None, [DP, I, I, I]),
+                             ("orc_cart_timesteps", None, [SP, D, D, D]),
+                             ("orc_cart_compute_P", None, [SP, D, D, D, D, D, D]),
+                             ("orc_cart_compute_TII", None, [SP]),
+                             ("orc_cart_power_law", None, [SP, D, D, D]),
+                             ("orc_cart_compute_V", None, [SP, D, D, D, D]),
+                             ("orc_cart_bc", None, [DP, I, I, I, I, DP, D]),
+                             ("orc_cart_check_err", D, [SP, DP]),
+                             ("orc_cart_iteration", None, [SP, PP, I]),
+                             ("orc_cart_pt_solve", I, [SP, PP, I, I, D, DP, C.POINTER(I)])):
+            f = getattr(L, n)
+            f.restype = res
+            f.argtypes = args
+        L._cart_sigs = True
+    return L
+
+
+class CartOracle:
+    """Oracle state of the Cartesian twin over numpy arrays in the
+    reference layout (Fortran order); `setup` is a
+    sphericalstokes_amd.cartesian.CartSetup (fields are copied)."""
+
+    def __init__(self, setup, fields=None):
+        self.L = _cart_lib()
+        global _DEFAULT_THREADS
+        if _DEFAULT_THREADS is None:
+            _DEFAULT_THREADS = self.L.orc_num_threads()
+        small = setup.nx * setup.ny * setup.nz < 24 ** 3
+        self.L.orc_set_num_threads(1 if small else _DEFAULT_THREADS)
+        src = fields if fields is not None else setup.fields
+        self.f = {k: np.array(v, dtype=np.float64, order="F", copy=True) for k, v in src.items()}
+        self.y = np.ascontiguousarray(setup.y, dtype=np.float64)
+        s = orc_cart()
+        s.nx, s.ny, s.nz = setup.nx, setup.ny, setup.nz
+        for m in _CART:
+            a = self.f[_CART_NAME.get(m, m)]
+            assert a.flags.f_contiguous
+            setattr(s, m, _ptr(a))
+        s.y1 = _ptr(self.y)
+        self.s = s
+        self.setup = setup
+
+    @staticmethod
+    def params(p):
+        return orc_cart_params(p._dx, p._dy, p._dz, p.dmp, p.vpdt, p.lx, p.tau_C, p.n_exp, p.relax, p.g, p.sh)
+
+    def equal(self, dst, src):
+        self.L.orc_cart_equal(_ptr(self.f[dst]), _ptr(self.f[src]), self.f[dst].size)
+
+    def maxloc(self):
+        self.L.orc_cart_maxloc(C.byref(self.s))
+
+    def copy_boundary(self, name):
+        a = self.f[name]
+        self.L.orc_cart_copy_boundary(_ptr(a), *a.shape)
+
+    def timesteps(self, vpdt, dmp, lx):
+        self.L.orc_cart_timesteps(C.byref(self.s), vpdt, dmp, lx)
+
+    def compute_P(self, dmp, vpdt, lx, _dx, _dy, _dz):
+        self.L.orc_cart_compute_P(C.byref(self.s), dmp, vpdt, lx, _dx, _dy, _dz)
+
+    def compute_TII(self):
+        self.L.orc_cart_compute_TII(C.byref(self.s))
+
+    def power_law(self, tau_C, n_exp, relax):
+        self.L.orc_cart_power_law(C.byref(self.s), tau_C, n_exp, relax)
+
+    def compute_V(self, _dx, _dy, _dz, g):
+        self.L.orc_cart_compute_V(C.byref(self.s), _dx, _dy, _dz, g)
+
+    def bc(self, op, name, fact=0.0):
+        a = self.f[name]
+        self.L.orc_cart_bc(_ptr(a), *a.shape, CART_BC[op], _ptr(self.y), float(fact))
+
+    def check_err(self):
+        comp = (C.c_double * 3)()
+        e = self.L.orc_cart_check_err(C.byref(self.s), comp)
+        return e, tuple(comp)
+
+    def step(self, p, n=1, check=False):
+        cp = self.params(p)
+        for _ in range(n):
+            self.L.orc_cart_iteration(C.byref(self.s), C.byref(cp), int(check))
+
+    def solve(self, p, maxiter, nout, eps):
+        err, conv = C.c_double(), C.c_int()
+        it = self.L.orc_cart_pt_solve(C.byref(self.s), C.byref(self.params(p)), int(maxiter), int(nout), float(eps),
+                                      C.byref(err), C.byref(conv))
+        return {"iters": it, "err": err.value, "converged": bool(conv.value)}
 
 
 def initialize_inclusion(A, R, th, ph, r, radius, val):

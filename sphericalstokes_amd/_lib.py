@@ -1,4 +1,4 @@
-"""ctypes binding of libsspt.so (the C ABI of include/sspt.h).
+"""ctypes binding of libsspt.so (the C ABI of include/sspt.h and include/sspt_cart.h).
 
 The library is loaded from this package directory.  There is no fallback:
 if it is missing or cannot be loaded, :func:`lib` raises.
@@ -62,7 +62,7 @@ class ss_adjoint_result(C.Structure):
 ADJ_FIELDS = ["VR", "Vth", "Vph", "P", "tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph", "eta",
               "eta_PL", "grad_rho", "grad_eta_ini"]
 
-# name -> (restype, argtypes); every symbol declared in include/sspt.h
+# name -> (restype, argtypes); every symbol declared in include/*.h
 P = C.c_void_p
 D = C.c_double
 I = C.c_int
@@ -109,6 +109,44 @@ SIGNATURES = {
     "ss_last_kernel_ms": (I, [P, DP]),
     "ss_pt_iterate_timed": (I, [P, C.POINTER(ss_params), I, DP, DP]),
 }
+
+
+class ss_cart_params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("_dx", "_dy", "_dz", "dmp", "vpdt", "lx", "tau_C", "n_exp", "relax",
+                                         "g", "sh")]
+
+
+class ss_cart_result(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("err", C.c_double), ("seconds", C.c_double)]
+
+
+# the Cartesian twin, include/sspt_cart.h
+SIGNATURES.update({
+    "ss_cart_create": (I, [I, I, I, I, C.POINTER(P)]),
+    "ss_cart_destroy": (None, [P]),
+    "ss_cart_last_error": (C.c_char_p, [P]),
+    "ss_cart_set_y": (I, [P, DP]),
+    "ss_cart_upload": (I, [P, I, DP, I]),
+    "ss_cart_download": (I, [P, I, DP, I]),
+    "ss_cart_equal": (I, [P, I, I]),
+    "ss_cart_maxloc": (I, [P]),
+    "ss_cart_copy_boundary": (I, [P, I]),
+    "ss_cart_timesteps": (I, [P, D, D, D]),
+    "ss_cart_compute_P": (I, [P, D, D, D, D, D, D]),
+    "ss_cart_compute_TII": (I, [P]),
+    "ss_cart_power_law": (I, [P, D, D, D]),
+    "ss_cart_compute_V": (I, [P, D, D, D, D]),
+    "ss_cart_bc_x_lin": (I, [P, I, D]),
+    "ss_cart_bc_x_0": (I, [P, I]),
+    "ss_cart_copy_bc_x": (I, [P, I]),
+    "ss_cart_bc_y_fact": (I, [P, I, D]),
+    "ss_cart_bc_y_0": (I, [P, I]),
+    "ss_cart_bc_z_0": (I, [P, I]),
+    "ss_cart_copy_bc_z": (I, [P, I]),
+    "ss_cart_check_err": (I, [P, DP, DP]),
+    "ss_cart_pt_solve": (I, [P, C.POINTER(ss_cart_params), I, I, D, C.POINTER(ss_cart_result)]),
+    "ss_cart_pt_iterate_timed": (I, [P, C.POINTER(ss_cart_params), I, DP, DP]),
+})
 
 _LIB = None
 

@@ -27,7 +27,13 @@ outputs):
   * maxloc: the etaSM part of compute_V_etaSM! (SphericalStokes_mxpu.jl);
   * init: initialize_inclusion! / initialize_velocity! (diapir, 15^3 sector
     grid, squared-distance test with radius 0.01) and initialize_geometry!
-    (plateau, on the script's scaled 70 x 5 x 5 grid and on 31 x 19 x 23).
+    (plateau, on the script's scaled 70 x 5 x 5 grid and on 31 x 19 x 23);
+  * cart_*: the Cartesian twin, scripts_diapir/CartesianStokes_diapir.jl --
+    every kernel one at a time (timesteps!, compute_P!, compute_TII!,
+    power_law! with relaxation 0.5, compute_V!, copy_boundary!, the
+    boundary kernels on the fields the loop applies them to, error!), three
+    iterations of its loop body (:351-379) with the exit-test copies on the
+    third, and initialize_inclusion! / initialize_velocity! (15^3).
 """
 from __future__ import annotations
 
@@ -147,6 +153,147 @@ def loop_body(S, mx, st, coords, sc, bc_args):
     run("BC_φ!", **bc_args)
 
 
+CART_SHAPES = {  # CartesianStokes_diapir.jl:317-337
+    "P": (0, 0, 0), "τ_XX": (0, 0, 0), "τ_YY": (0, 0, 0), "τ_ZZ": (0, 0, 0),
+    "τ_XY": (-1, -1, -2), "τ_XZ": (-1, -2, -1), "τ_YZ": (-2, -1, -1),
+    "VX": (1, 0, 0), "VY": (0, 1, 0), "VZ": (0, 0, 1), "ERR_VX": (1, 0, 0), "ERR_VY": (0, 1, 0),
+    "ERR_VZ": (0, 0, 1), "ρ": (0, 0, 0), "η": (0, 0, 0), "η_INI": (0, 0, 0), "η_PL": (0, 0, 0),
+    "η_PL_OLD": (0, 0, 0), "ηSM": (0, 0, 0), "DT_R": (0, 0, 0), "τII": (0, 0, 0),
+}
+CART_FIELD = {"τ_XX": "tau_XX", "τ_YY": "tau_YY", "τ_ZZ": "tau_ZZ", "τ_XY": "tau_XY", "τ_XZ": "tau_XZ",
+              "τ_YZ": "tau_YZ", "ρ": "rho", "η": "eta", "η_INI": "eta_INI", "η_PL": "eta_PL",
+              "η_PL_OLD": "eta_PL_OLD", "ηSM": "etaSM", "τII": "tII"}
+
+
+def cart_vectors(cart, arrays, meta):
+    """Kernels of CartesianStokes_diapir.jl one at a time on a seeded random
+    state (7 x 6 x 8 cells, lx = ly = lz = 6), three iterations of its loop
+    body (:351-379) with the exit-test copies and error! on the third, and
+    its setup kernels on a 15^3 grid."""
+    nx, ny, nz = 7, 6, 8
+    lx = ly = lz = 6.0
+    x = np.linspace(-lx / 2.0, lx / 2.0, nx)
+    y = np.linspace(-ly / 2.0, ly / 2.0, ny)
+    z = np.linspace(-lz / 2.0, lz / 2.0, nz)
+    vr = 1e2
+    CFL = 1.0 / (2.0 + 4.5 * math.log10(vr))
+    dx = lx / (nx - 1)
+    sc = dict(_dx=1.0 / dx, _dy=1.0 / (ly / (ny - 1)), _dz=1.0 / (lz / (nz - 1)), dmp=4.5, vpdt=dx * CFL, lx=lx,
+              τ_C=1.0, n_exp=3.0, relax=0.5, g=1.0, sh=1.0)
+    g = np.random.default_rng(23)
+    st0 = {}
+    for name, d in CART_SHAPES.items():
+        shp = (nx + d[0], ny + d[1], nz + d[2])
+        if name in ("η", "η_INI", "η_PL", "η_PL_OLD", "ηSM", "DT_R", "τII"):
+            a = 10.0 ** g.normal(0.0, 0.5, shp)
+        else:
+            a = g.normal(0.0, 1.0, shp)
+        st0[name] = np.asfortranarray(a)
+    coords = {"Y_VX": np.asfortranarray(np.broadcast_to(y[None, :, None], (nx + 1, ny, nz)).copy()),
+              "Y_VZ": np.asfortranarray(np.broadcast_to(y[None, :, None], (nx, ny, nz + 1)).copy())}
+    meta["cart"] = {"grid": [nx, ny, nz], "scalars": {CART_FIELD.get(k, k).replace("τ_C", "tau_C"): v
+                                                      for k, v in sc.items()}}
+    meta["cart"]["scalars"]["tau_C"] = sc["τ_C"]
+    meta["cart"]["scalars"].pop("τ_C", None)
+    arrays["cart__x"], arrays["cart__y"], arrays["cart__z"] = x, y, z
+
+    def put(case, name, a):
+        arrays[f"cart_{case}__out__{CART_FIELD.get(name, name)}"] = np.asarray(a)
+
+    for n, a in st0.items():
+        arrays[f"cart_in__state__{CART_FIELD.get(n, n)}"] = a
+
+    def run(st, kname, **over):
+        k = cart.kernels[kname]
+        env = {**sc, **coords, **st, **over}
+        cart.run(kname, *[env[p] for p in k.params])
+
+    one = {"timesteps!": ["DT_R"], "compute_P!": ["P", "τ_XX", "τ_YY", "τ_ZZ", "τ_XY", "τ_XZ", "τ_YZ"],
+           "compute_TII!": ["τII"], "power_law!": ["η_PL", "η"], "compute_V!": ["VX", "VY", "VZ"]}
+    for kname, outs in one.items():
+        st = copy(st0)
+        run(st, kname)
+        for n in outs:
+            assert not np.array_equal(st[n], st0[n]), (kname, n)
+            put(kname.rstrip("!"), n, st[n])
+    st = copy(st0)
+    run(st, "copy_boundary!", A=st["τII"])
+    put("copy_boundary_tII", "τII", st["τII"])
+    bcs = [("bc_x_lin!", "VX", "Y_VX", "sh"), ("bc_x_0!", "VY", None, None), ("copy_bc_x!", "VZ", None, None),
+           ("bc_y_fact!", "VX", "Y_VX", "sh"), ("bc_y_fact!", "VZ", "Y_VZ", 0.0), ("bc_y_0!", "VY", None, None),
+           ("bc_z_0!", "VZ", None, None), ("copy_bc_z!", "VX", None, None)]
+    for kname, fld, coord, fact in bcs:
+        st = copy(st0)
+        over = {"A": st[fld]}
+        if coord is not None:
+            over["COORD"] = coords[coord]
+            over["fact"] = sc["sh"] if fact == "sh" else fact
+        run(st, kname, **over)
+        put(f"{kname.rstrip('!')}_{fld}", fld, st[fld])
+    st = copy(st0)
+    for v in ("VX", "VY", "VZ"):
+        run(st, "error!", A=st["ERR_" + v], B=st[v])
+        put("error", "ERR_" + v, st["ERR_" + v])
+
+    # three iterations of the loop body, the exit-test copies and error! on the third
+    st = copy(st0)
+    for it in (1, 2, 3):
+        run(st, "equal!", A=st["ηSM"], B=st["η"])
+        eta, sm = st["η"], st["ηSM"]      # maxloc!: @inn(etaSM) = @maxloc(eta)
+        sm[1:-1, 1:-1, 1:-1] = _maxloc_inner(eta)
+        run(st, "copy_boundary!", A=st["ηSM"])
+        run(st, "timesteps!")
+        run(st, "compute_P!")
+        run(st, "compute_TII!")
+        run(st, "copy_boundary!", A=st["τII"])
+        run(st, "equal!", A=st["η_PL_OLD"], B=st["η_PL"])
+        run(st, "power_law!")
+        if it == 3:
+            for v in ("VX", "VY", "VZ"):
+                run(st, "equal!", A=st["ERR_" + v], B=st[v])
+        run(st, "compute_V!")
+        run(st, "bc_x_lin!", A=st["VX"], COORD=coords["Y_VX"], fact=sc["sh"])
+        run(st, "bc_x_0!", A=st["VY"])
+        run(st, "bc_x_0!", A=st["VZ"])
+        run(st, "bc_y_fact!", A=st["VX"], COORD=coords["Y_VX"], fact=sc["sh"])
+        run(st, "bc_y_0!", A=st["VY"])
+        run(st, "bc_y_fact!", A=st["VZ"], COORD=coords["Y_VZ"], fact=0.0)
+        run(st, "copy_bc_z!", A=st["VX"])
+        run(st, "copy_bc_z!", A=st["VY"])
+        run(st, "bc_z_0!", A=st["VZ"])
+        if it == 3:
+            for v in ("VX", "VY", "VZ"):
+                run(st, "error!", A=st["ERR_" + v], B=st[v])
+    for n in CART_SHAPES:
+        put("loop", n, st[n])
+
+    # setup kernels, 15^3 (radius 1, vr 1e2, sh 1, as the script)
+    n = 15
+    xs = np.linspace(-3.0, 3.0, n)
+    X = np.asfortranarray(np.broadcast_to(xs[:, None, None], (n, n, n)).copy())
+    Y = np.asfortranarray(np.broadcast_to(xs[None, :, None], (n, n, n)).copy())
+    Z = np.asfortranarray(np.broadcast_to(xs[None, None, :], (n, n, n)).copy())
+    A = np.asfortranarray(np.ones((n, n, n)))
+    cart.run("initialize_inclusion!", A, X, Y, Z, 1.0, 0.01)
+    assert 0 < np.count_nonzero(A != 1.0) < A.size
+    arrays["cart_init__out__inclusion"] = A
+    V = np.asfortranarray(np.zeros((n + 1, n, n)))
+    cart.run("initialize_velocity!", V, Y, 1.0)
+    arrays["cart_init__out__velocity"] = V
+    arrays["cart_init__x"] = xs
+
+
+def _maxloc_inner(eta):
+    """@maxloc(eta) on the inner points (ParallelStencil's nesting, the
+    expansion written out in SphericalStokes_mxpu.jl:295-297); Julia's max
+    propagates NaN, the random state has none."""
+    c = eta[1:-1, 1:-1, 1:-1]
+    m = np.maximum(np.maximum(np.maximum(np.maximum(eta[:-2, 1:-1, 1:-1], eta[2:, 1:-1, 1:-1]), c),
+                              np.maximum(eta[1:-1, :-2, 1:-1], eta[1:-1, 2:, 1:-1])),
+                   np.maximum(eta[1:-1, 1:-1, :-2], eta[1:-1, 1:-1, 2:]))
+    return m
+
+
 def main():
     diapir = J.Script((REF / "scripts_diapir" / "SphericalStokes_diapir.jl").read_text())
     plateau = J.Script((REF / "scripts_plateau" / "SphericalStokes_plateau.jl").read_text())
@@ -247,6 +394,9 @@ def main():
             arrays[f"{tag}__out__{what}"] = A
         arrays[f"{tag}__R1"], arrays[f"{tag}__th1"], arrays[f"{tag}__ph1"] = su.R, su.th, su.ph
         meta[tag] = {"grid": [gnr, gnt, gnp], **{k: float(v) for k, v in gp.items()}}
+
+    # ---- the Cartesian twin (CartesianStokes_diapir.jl)
+    cart_vectors(cart, arrays, meta)
 
     np.savez_compressed(OUT.with_suffix(".npz"), **arrays)
     OUT.with_suffix(".json").write_text(json.dumps(meta, indent=1, sort_keys=True) + "\n")

@@ -1,4 +1,4 @@
-"""The C ABI: libsspt.so loads and exports every symbol include/sspt.h
+"""The C ABI: libsspt.so loads and exports every symbol include/*.h
 declares; argument checks that need no GPU return SS_ERR_ARG."""
 import ctypes as C
 import re
@@ -12,8 +12,11 @@ ROOT = Path(__file__).resolve().parent.parent
 
 
 def declared_symbols():
-    txt = (ROOT / "include" / "sspt.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(ss_\w+)\s*\(", txt, flags=re.M)))
+    syms = set()
+    for h in sorted((ROOT / "include").glob("*.h")):
+        txt = h.read_text()
+        syms |= set(re.findall(r"^\s*(?:int|void|const char \*)\s*(ss_\w+)\s*\(", txt, flags=re.M))
+    return sorted(syms)
 
 
 def test_header_symbols_exported():
@@ -35,12 +38,19 @@ def test_create_rejects_bad_sizes_without_gpu():
     assert L.ss_create(10, 10, 10, 0, None) == _lib.SS_ERR_ARG
     assert L.ss_destroy(None) == _lib.SS_ERR_ARG
     assert L.ss_last_error(None) == b"null context"
+    assert L.ss_cart_create(2, 10, 10, 0, C.byref(h)) == _lib.SS_ERR_ARG
+    assert L.ss_cart_create(10, 10, 10, 0, None) == _lib.SS_ERR_ARG
+    assert L.ss_cart_last_error(None) == b"null context"
+    L.ss_cart_destroy(None)
+    assert L.ss_cart_maxloc(None) == _lib.SS_ERR_ARG
 
 
 def test_params_struct_layout():
     # ss_params: 14 doubles + 3 ints, padded to 8 bytes
     assert C.sizeof(_lib.ss_params) == 14 * 8 + 4 * 4
     assert C.sizeof(_lib.ss_result) == 4 + 4 + 8 + 24 + 8
+    assert C.sizeof(_lib.ss_cart_params) == 11 * 8
+    assert C.sizeof(_lib.ss_cart_result) == 4 + 4 + 8 + 8
 
 
 def test_check_err_noalloc_rejects_without_gpu():

@@ -1,0 +1,205 @@
+"""GPU: the Cartesian twin (include/sspt_cart.h) against the reference's
+kernel text, the Cartesian oracle, and the spherical solver in the
+thin-sector limit.
+
+* every kernel entry point equals the vectors evaluated from
+  CartesianStokes_diapir.jl bit for bit (power_law! within 8 ulp: the
+  device's pow, exp and log against the C library's), three loop
+  iterations within rel-L2 1e-12;
+* the same kernels against the oracle on a larger random state, and the
+  whole loop with its exit test on the script's setup (linear: the same
+  iteration count and err as the oracle, bit for bit; n_exp = 5 with
+  relaxation 1e-3 as the script: 300 iterations within rel-L2 1e-10);
+* cross-coordinate check (docs/fig_compare.png): the spherical diapir of
+  SphericalStokes_diapir.jl (r = 1000, lr = 6, lth = lph = lr/r, simple
+  shear) and the Cartesian box (lx = 6) solve the same Stokes problem up to
+  metric terms of order lr/r; their converged velocities agree to O(lr/r),
+  and ten times closer at r = 10000.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import max_ulp, rel_l2
+from oracle.binding import CartOracle
+from sphericalstokes_amd.cartesian import CART_FIELDS, CartSolver, cart_diapir
+from test_cart_reference_text import FIX, KERNEL_NAMES, _kernels, cart_text_setup
+
+pytestmark = pytest.mark.gpu
+ULP_POW = 8
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _gpu(st):
+    s = CartSolver(st.nx, st.ny, st.nz)
+    s.load_setup(st)
+    return s
+
+
+class _Dev:
+    """Adapter: the oracle-style kernel calls of _kernels() on a CartSolver."""
+
+    def __init__(self, s):
+        self.s = s
+
+    def __getattr__(self, n):
+        return getattr(self.s, n)
+
+    def bc(self, op, name, fact=None):
+        f = getattr(self.s, op)
+        return f(name, fact) if op in ("bc_x_lin", "bc_y_fact") else f(name)
+
+
+@pytest.mark.parametrize("kernel", KERNEL_NAMES)
+def test_cart_kernel_equals_reference_text(kernel):
+    st = cart_text_setup()
+    say("cart reference text:", kernel, (st.nx, st.ny, st.nz))
+    with _gpu(st) as s:
+        _kernels(st.params)[kernel](_Dev(s))
+        bad = {}
+        for k in FIX.files:
+            if not k.startswith(f"cart_{kernel}__out__"):
+                continue
+            n = k.split("__out__")[1]
+            got, ref = s.download(n), FIX[k]
+            if kernel == "power_law":
+                u = max_ulp(got, ref)
+                if u > ULP_POW:
+                    bad[n] = f"{u} ulp"
+            elif not np.array_equal(got.view(np.uint64), ref.view(np.uint64)):
+                bad[n] = int(np.count_nonzero(got != ref))
+        assert not bad, bad
+
+
+def test_cart_loop_equals_reference_text():
+    st = cart_text_setup()
+    say("cart reference text: loop")
+    with _gpu(st) as s:
+        s.iteration(st.params)
+        s.iteration(st.params)
+        s.iteration(st.params, check=True)
+        s.check_err()
+        worst = 0.0
+        for k in FIX.files:
+            if k.startswith("cart_loop__out__"):
+                n = k.split("__out__")[1]
+                e = rel_l2(s.download(n), FIX[k])
+                worst = max(worst, e)
+                assert e <= 1e-12, (n, e)
+        say("  worst rel-L2", worst)
+
+
+def _random_cart(shape, seed):
+    s = cart_diapir(*shape, n_exp=3.0, relax=0.5)
+    g = np.random.default_rng(seed)
+    for n in CART_FIELDS:
+        a = s.fields[n]
+        if n in ("eta", "eta_INI", "eta_PL", "eta_PL_OLD", "etaSM", "DT_R", "tII"):
+            a[...] = 10.0 ** g.normal(0.0, 0.5, a.shape)
+        else:
+            a[...] = g.normal(0.0, 1.0, a.shape)
+    return s
+
+
+@pytest.mark.parametrize("kernel", KERNEL_NAMES)
+def test_cart_kernel_matches_oracle(kernel):
+    st = _random_cart((21, 40, 70), seed=5)
+    say("cart vs oracle:", kernel, (st.nx, st.ny, st.nz))
+    o = CartOracle(st)
+    _kernels(st.params)[kernel](o)
+    with _gpu(st) as s:
+        _kernels(st.params)[kernel](_Dev(s))
+        for n in CART_FIELDS:
+            got = s.download(n)
+            if kernel == "power_law" and n in ("eta", "eta_PL"):
+                assert max_ulp(got, o.f[n]) <= ULP_POW, n
+            else:
+                assert np.array_equal(got, o.f[n]), (kernel, n, int(np.count_nonzero(got != o.f[n])))
+
+
+def test_cart_check_err_matches_oracle():
+    st = _random_cart((13, 17, 66), seed=9)
+    o = CartOracle(st)
+    e_o, c_o = o.check_err()
+    with _gpu(st) as s:
+        e_g, c_g = s.check_err()
+        for v in ("VX", "VY", "VZ"):
+            assert np.array_equal(s.download("ERR_" + v), o.f["ERR_" + v]), v
+    assert c_g == c_o and e_g == e_o
+
+
+def test_cart_solve_linear_matches_oracle_bitwise():
+    """The script's setup with n_exp = 1 at 31^3: the loop and its exit test
+    give the oracle's iteration count, err and fields bit for bit."""
+    st = cart_diapir(31, n_exp=1.0)
+    o = CartOracle(st)
+    ro = o.solve(st.params, 200_000, 100, 5e-7)
+    say("cart 31^3 linear: oracle", ro)
+    with _gpu(st) as s:
+        rg = s.pt_solve(st.params, 200_000, 100, 5e-7)
+        say("cart 31^3 linear: gpu", rg)
+        assert rg["converged"] and ro["converged"]
+        assert rg["iters"] == ro["iters"] and rg["err"] == ro["err"]
+        for n in CART_FIELDS:
+            assert np.array_equal(s.download(n), o.f[n]), n
+
+
+def test_cart_powerlaw_iterations_match_oracle():
+    """The script's own parameters (n_exp = 5, relaxation 1e-3) at 31^3:
+    300 iterations within rel-L2 1e-10 of the oracle."""
+    st = cart_diapir(31)
+    o = CartOracle(st)
+    o.step(st.params, 300)
+    with _gpu(st) as s:
+        for _ in range(300):
+            s.iteration(st.params)
+        worst = 0.0
+        for n in CART_FIELDS:
+            e = rel_l2(s.download(n), o.f[n])
+            worst = max(worst, e)
+            assert e <= 1e-10, (n, e)
+    say("cart 31^3 n=5: worst field rel-L2 after 300 iterations", worst)
+
+
+def _spherical_thin(n, r):
+    """SPH_3D_Inclusion's setup (SphericalStokes_diapir.jl:350-404) at n^3
+    with linear viscosity: r = sector radius, lr = 6, lth = lph = lr/r."""
+    from sphericalstokes_amd import setups
+    return setups.diapir(n, n, n, n_exp=1.0, r=r, lr=6.0, radius=1.0, sh=1.0, vr=1e2, drho=-10.0)
+
+
+def _solve_both(n, r, eps):
+    from sphericalstokes_amd.solver import Solver
+    sp = _spherical_thin(n, r)
+    with Solver(n, n, n) as s:
+        s.load_setup(sp)
+        res = s.pt_solve(sp.params, 2_000_000, 100, eps)
+        vs = {k: s.download(k) for k in ("VR", "Vth", "Vph")}
+    ct = cart_diapir(n, n_exp=1.0)
+    with _gpu(ct) as c:
+        rc = c.pt_solve(ct.params, 2_000_000, 100, eps)
+        vc = {k: c.download(k) for k in ("VX", "VY", "VZ")}
+    # spherical arrays are (R, th, ph) = (z, y, x): transpose onto (x, y, z)
+    pairs = {"VX": vs["Vph"].transpose(2, 1, 0), "VY": vs["Vth"].transpose(2, 1, 0), "VZ": vs["VR"].transpose(2, 1, 0)}
+    err = {k: float(np.linalg.norm(pairs[k] - vc[k]) / np.linalg.norm(vc[k])) for k in pairs}
+    return res, rc, err
+
+
+def test_cross_coordinate_thin_sector_approaches_cartesian():
+    """docs/fig_compare.png's check: the spherical diapir in a thin sector
+    and the Cartesian diapir, same Stokes problem up to metric terms of
+    order lr/r.  Converged velocities: relative difference O(lr/r) at
+    r = 1000 and about ten times smaller at r = 10000."""
+    n, eps = 23, 1e-10
+    res1, rc1, e1 = _solve_both(n, 1000.0, eps)
+    say("cross-coordinate r=1000:", res1, rc1, e1)
+    res2, rc2, e2 = _solve_both(n, 10000.0, eps)
+    say("cross-coordinate r=10000:", res2, rc2, e2)
+    for k in ("VX", "VY", "VZ"):
+        assert e1[k] < 3e-2, (k, e1[k])           # lr/r = 6e-3
+        assert e2[k] < 0.2 * e1[k] + 1e-6, (k, e1[k], e2[k])
