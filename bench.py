@@ -22,6 +22,7 @@ the linear diapir_sector instead.
 "adjoint" holds the cost of one adjoint iteration on the same workload
 against one FP64 iteration, and the wall times of a forward solve and of
 the adjoint solve that gives the gradient (power-law diapir at --adjoint-n^3).
+"cartesian_twin" times the Cartesian twin of the diapir solver at --cart-n^3.
 """
 from __future__ import annotations
 
@@ -216,6 +217,38 @@ def time_to_tolerance(n, tol, tol_inner, setup="diapir"):
             "P_rel_err_mixed_vs_fp64": float("%.3e" % ehist[-1][1])}
 
 
+def cartesian_twin(n, steps, warmup=3):
+    """The Cartesian twin (CartesianStokes_diapir.jl, the script's own
+    parameters: n_exp = 5, relaxation 1e-3, simple shear) at n^3: one
+    iteration is the script's kernel sequence (:351-379, about 20 launches).
+    T_eff with the same A_eff accounting as the spherical path; the roofline
+    of its dominant kernel compute_P!, whose algorithmic bytes per cell are
+    19 x 8 (reads VX VY VZ eta etaSM, reads and writes P and the six
+    stresses)."""
+    import numpy as np
+    from sphericalstokes_amd.cartesian import CART_FIELDS, CartSolver, cart_diapir
+
+    st = cart_diapir(n)
+    p = st.params
+    with CartSolver(n, n, n) as s:
+        s.set_y(st.y)
+        for name in CART_FIELDS:          # the context starts at zero
+            a = st.fields.pop(name)
+            if np.any(a):
+                s.upload(name, a)
+            del a
+        s.pt_iterate_timed(p, warmup)
+        it_ms, k_ms = s.pt_iterate_timed(p, steps)
+    A = a_eff_bytes(n)
+    kb = 19 * 8 * n ** 3
+    return {"workload": f"cartesian diapir {n}^3 FP64 (CartesianStokes_diapir.jl setup: n_exp=5, relax 1e-3, "
+                        f"simple shear), per-kernel launches", "iter_ms": round(it_ms, 4),
+            "T_eff_GBps": round(A / (it_ms * 1e-3) / 1e9, 2),
+            "compute_P": {"kernel_ms": round(k_ms, 4), "bytes": kb,
+                          "achieved_GBps": round(kb / (k_ms * 1e-3) / 1e9, 2),
+                          "frac_of_peak": round(kb / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}}
+
+
 def hbm_measured():
     """Streaming rates of plain torch kernels, measured in a child process.
 
@@ -350,6 +383,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain field copy (k_copy_field: n^3 doubles read + written) so that "
                          "rocprofv3 --pmc byte counters can be scaled")
+    ap.add_argument("--cart-n", type=int, default=511,
+                    help="size of the Cartesian twin's timing leg; 0 skips it")
     ap.add_argument("--hbm-probe-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.hbm_probe_child:        # the child process of hbm_measured()
@@ -418,6 +453,8 @@ def main():
     if args.ttt_n > 0:
         out["mixed_precision"]["time_to_tol"] = time_to_tolerance(args.ttt_n, args.ttt_tol, args.ttt_tol_inner,
                                                                   args.ttt_setup)
+    if args.cart_n > 0:
+        out["cartesian_twin"] = cartesian_twin(args.cart_n, 10)
     if not args.no_hbm_probe:
         out["hbm_measured"] = hbm_measured()
     if args.mixed_large_n > 0:
