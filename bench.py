@@ -22,7 +22,8 @@ the linear diapir_sector instead.
 "adjoint" holds the cost of one adjoint iteration on the same workload
 against one FP64 iteration, and the wall times of a forward solve and of
 the adjoint solve that gives the gradient (power-law diapir at --adjoint-n^3).
-"cartesian_twin" times the Cartesian twin of the diapir solver at --cart-n^3.
+"cartesian_twin" / "cylindrical_twin" time the Cartesian and cylindrical twins
+of the diapir solver at --cart-n^3 / --cyl-n^3.
 """
 from __future__ import annotations
 
@@ -249,6 +250,38 @@ def cartesian_twin(n, steps, warmup=3):
                           "frac_of_peak": round(kb / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}}
 
 
+def cylindrical_twin(n, steps, warmup=3):
+    """The cylindrical twin (CylindricalStokes_diapir.jl, the script's setup:
+    thin shell r = 1000, linear phase n_exp = 1) at n^3: one iteration is the
+    script's kernel sequence (:331-366, about 25 launches).  T_eff by the
+    spherical path's A_eff accounting; the roofline of compute_P!, whose
+    algorithmic bytes per cell are 24 x 8 (reads VR Vth VZ KBDT GSDT eta,
+    reads and writes P and the six stresses, writes DIVV and the six strain
+    rates)."""
+    import numpy as np
+    from sphericalstokes_amd.cylindrical import CYL_FIELDS, CylSolver, cyl_diapir
+
+    st = cyl_diapir(n)
+    p = st.params
+    with CylSolver(n, n, n) as s:
+        s.set_grid(st.R, st.th)
+        for name in CYL_FIELDS:           # the context starts at zero
+            a = st.fields.pop(name)
+            if np.any(a):
+                s.upload(name, a)
+            del a
+        s.pt_iterate_timed(p, warmup)
+        it_ms, k_ms = s.pt_iterate_timed(p, steps)
+    A = a_eff_bytes(n)
+    kb = 27 * 8 * n ** 3
+    return {"workload": f"cylindrical diapir {n}^3 FP64 (CylindricalStokes_diapir.jl setup, linear phase), "
+                        f"per-kernel launches", "iter_ms": round(it_ms, 4),
+            "T_eff_GBps": round(A / (it_ms * 1e-3) / 1e9, 2),
+            "compute_P": {"kernel_ms": round(k_ms, 4), "bytes": kb,
+                          "achieved_GBps": round(kb / (k_ms * 1e-3) / 1e9, 2),
+                          "frac_of_peak": round(kb / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}}
+
+
 def hbm_measured():
     """Streaming rates of plain torch kernels, measured in a child process.
 
@@ -385,6 +418,8 @@ def main():
                          "rocprofv3 --pmc byte counters can be scaled")
     ap.add_argument("--cart-n", type=int, default=511,
                     help="size of the Cartesian twin's timing leg; 0 skips it")
+    ap.add_argument("--cyl-n", type=int, default=511,
+                    help="size of the cylindrical twin's timing leg; 0 skips it")
     ap.add_argument("--hbm-probe-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.hbm_probe_child:        # the child process of hbm_measured()
@@ -455,6 +490,8 @@ def main():
                                                                   args.ttt_setup)
     if args.cart_n > 0:
         out["cartesian_twin"] = cartesian_twin(args.cart_n, 10)
+    if args.cyl_n > 0:
+        out["cylindrical_twin"] = cylindrical_twin(args.cyl_n, 10)
     if not args.no_hbm_probe:
         out["hbm_measured"] = hbm_measured()
     if args.mixed_large_n > 0:

@@ -1,5 +1,5 @@
 """ctypes binding of the CPU oracle (oracle/libsspt_oracle.so): the spherical
-iteration, its adjoint and the Cartesian twin.
+iteration, its adjoint and the Cartesian and cylindrical twins.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, as the checker.  The product never uses it.
@@ -335,6 +335,124 @@ class CartOracle:
         err, conv = C.c_double(), C.c_int()
         it = self.L.orc_cart_pt_solve(C.byref(self.s), C.byref(self.params(p)), int(maxiter), int(nout), float(eps),
                                       C.byref(err), C.byref(conv))
+        return {"iters": it, "err": err.value, "converged": bool(conv.value)}
+
+
+# ---- the cylindrical twin (oracle/sspt_cyl_oracle.c)
+CYL = ["P", "DIVV", "VR", "Vth", "VZ", "dVR", "dVth", "dVZ", "VR_OLD", "Vth_OLD", "VZ_OLD", "D_RR", "D_thth",
+       "D_ZZ", "D_Rth", "D_RZ", "D_thZ", "tau_RR", "tau_thth", "tau_ZZ", "tau_Rth", "tau_RZ", "tau_thZ", "sig_RR",
+       "sig_thth", "sig_ZZ", "rho", "rhoG", "eta", "eta_INI", "eta_PL", "eta_PL_OLD", "etaSM", "KBDT", "GSDT", "DT_R",
+       "tII", "ERR_Rth"]
+CYL_BC = {"copy_bc_r": 0, "copy_bc_th": 1, "copy_bc_z": 2, "bc_r_0": 3, "bc_th_fact": 4, "bc_th_0": 5, "bc_z_0": 6,
+          "bc_z_lin": 7}
+
+
+class orc_cyl(C.Structure):
+    _fields_ = [("nr", C.c_int), ("nth", C.c_int), ("nz", C.c_int), ("f", DP * len(CYL)), ("R1", DP), ("th1", DP)]
+
+
+class orc_cyl_params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dr", "dth", "dz", "dmp", "vpdt", "lr", "tau_C", "n_exp", "n_exp_PL",
+                                         "relax", "g", "sh", "r")]
+
+
+def _cyl_lib():
+    L = lib()
+    if not getattr(L, "_cyl_sigs", False):
+        SP, PP, I, D = C.POINTER(orc_cyl), C.POINTER(orc_cyl_params), C.c_int, C.c_double
+        for n, res, args in (("orc_cyl_equal", None, [SP, I, I]), ("orc_cyl_multiply", None, [SP, I, I, D]),
+                             ("orc_cyl_maxloc", None, [SP]), ("orc_cyl_timesteps", None, [SP, D, D, D]),
+                             ("orc_cyl_timesteps2", None, [SP, D]), ("orc_cyl_compute_P", None, [SP, D, D, D]),
+                             ("orc_cyl_compute_sigma", None, [SP]), ("orc_cyl_compute_TII", None, [SP]),
+                             ("orc_cyl_power_law", None, [SP, D, D, D]), ("orc_cyl_compute_dV", None, [SP, D, D, D]),
+                             ("orc_cyl_compute_V", None, [SP]), ("orc_cyl_bc", None, [SP, I, I, D, D]),
+                             ("orc_cyl_err_rt", None, [SP]), ("orc_cyl_exit_error", D, [SP, DP]),
+                             ("orc_cyl_iteration", None, [SP, PP, D]),
+                             ("orc_cyl_pt_solve", I, [SP, PP, I, I, D, DP, C.POINTER(I)])):
+            f = getattr(L, n)
+            f.restype = res
+            f.argtypes = args
+        L._cyl_sigs = True
+    return L
+
+
+class CylOracle:
+    """Oracle state of the cylindrical twin (numpy arrays, reference layout);
+    `setup` is a sphericalstokes_amd.cylindrical.CylSetup (fields copied)."""
+
+    def __init__(self, setup, fields=None):
+        self.L = _cyl_lib()
+        src = fields if fields is not None else setup.fields
+        self.f = {k: np.array(v, dtype=np.float64, order="F", copy=True) for k, v in src.items()}
+        self.R = np.ascontiguousarray(setup.R, dtype=np.float64)
+        self.th = np.ascontiguousarray(setup.th, dtype=np.float64)
+        s = orc_cyl()
+        s.nr, s.nth, s.nz = setup.nr, setup.nth, setup.nz
+        for q, n in enumerate(CYL):
+            assert self.f[n].flags.f_contiguous
+            s.f[q] = _ptr(self.f[n])
+        s.R1, s.th1 = _ptr(self.R), _ptr(self.th)
+        self.s = s
+        self.setup = setup
+
+    @staticmethod
+    def params(p):
+        return orc_cyl_params(p.dr, p.dth, p.dz, p.dmp, p.vpdt, p.lr, p.tau_C, p.n_exp, p.n_exp_PL, p.relax, p.g,
+                              p.sh, p.r)
+
+    def equal(self, dst, src):
+        self.L.orc_cyl_equal(C.byref(self.s), CYL.index(dst), CYL.index(src))
+
+    def multiply(self, dst, src, fact):
+        self.L.orc_cyl_multiply(C.byref(self.s), CYL.index(dst), CYL.index(src), float(fact))
+
+    def maxloc(self):
+        self.L.orc_cyl_maxloc(C.byref(self.s))
+
+    def timesteps(self, dmp, vpdt, lr):
+        self.L.orc_cyl_timesteps(C.byref(self.s), dmp, vpdt, lr)
+
+    def timesteps2(self, vpdt):
+        self.L.orc_cyl_timesteps2(C.byref(self.s), vpdt)
+
+    def compute_P(self, dr, dth, dz):
+        self.L.orc_cyl_compute_P(C.byref(self.s), dr, dth, dz)
+
+    def compute_sigma(self):
+        self.L.orc_cyl_compute_sigma(C.byref(self.s))
+
+    def compute_TII(self):
+        self.L.orc_cyl_compute_TII(C.byref(self.s))
+
+    def power_law(self, tau_C, n_exp, relax):
+        self.L.orc_cyl_power_law(C.byref(self.s), tau_C, n_exp, relax)
+
+    def compute_dV(self, dr, dth, dz):
+        self.L.orc_cyl_compute_dV(C.byref(self.s), dr, dth, dz)
+
+    def compute_V(self):
+        self.L.orc_cyl_compute_V(C.byref(self.s))
+
+    def bc(self, op, name, fact=0.0, r=0.0):
+        self.L.orc_cyl_bc(C.byref(self.s), CYL.index(name), CYL_BC[op], float(fact), float(r))
+
+    def err_rth(self):
+        self.L.orc_cyl_err_rt(C.byref(self.s))
+
+    def exit_error(self):
+        comp = (C.c_double * 3)()
+        e = self.L.orc_cyl_exit_error(C.byref(self.s), comp)
+        return e, tuple(comp)
+
+    def step(self, p, n=1, n_exp=None):
+        cp = self.params(p)
+        for _ in range(n):
+            self.L.orc_cyl_iteration(C.byref(self.s), C.byref(cp), float(p.n_exp if n_exp is None else n_exp))
+
+    def solve(self, p, maxiter, nout, eps):
+        err, conv = C.c_double(), C.c_int()
+        it = self.L.orc_cyl_pt_solve(C.byref(self.s), C.byref(self.params(p)), int(maxiter), int(nout), float(eps),
+                                     C.byref(err), C.byref(conv))
         return {"iters": it, "err": err.value, "converged": bool(conv.value)}
 
 

@@ -1,4 +1,4 @@
-"""ctypes binding of libsspt.so (the C ABI of include/sspt.h and include/sspt_cart.h).
+"""ctypes binding of libsspt.so (the C ABI of include/sspt.h, sspt_cart.h and sspt_cyl.h).
 
 The library is loaded from this package directory.  There is no fallback:
 if it is missing or cannot be loaded, :func:`lib` raises.
@@ -146,6 +146,43 @@ SIGNATURES.update({
     "ss_cart_check_err": (I, [P, DP, DP]),
     "ss_cart_pt_solve": (I, [P, C.POINTER(ss_cart_params), I, I, D, C.POINTER(ss_cart_result)]),
     "ss_cart_pt_iterate_timed": (I, [P, C.POINTER(ss_cart_params), I, DP, DP]),
+})
+
+
+class ss_cyl_params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dr", "dth", "dz", "dmp", "vpdt", "lr", "tau_C", "n_exp", "n_exp_PL",
+                                         "relax", "g", "sh", "r")]
+
+
+class ss_cyl_result(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("err", C.c_double), ("n_exp", C.c_double),
+                ("seconds", C.c_double)]
+
+
+# the cylindrical twin, include/sspt_cyl.h
+SIGNATURES.update({
+    "ss_cyl_create": (I, [I, I, I, I, C.POINTER(P)]),
+    "ss_cyl_destroy": (None, [P]),
+    "ss_cyl_last_error": (C.c_char_p, [P]),
+    "ss_cyl_set_grid": (I, [P, DP, DP]),
+    "ss_cyl_upload": (I, [P, I, DP, I]),
+    "ss_cyl_download": (I, [P, I, DP, I]),
+    "ss_cyl_equal": (I, [P, I, I]),
+    "ss_cyl_multiply": (I, [P, I, I, D]),
+    "ss_cyl_maxloc": (I, [P]),
+    "ss_cyl_timesteps": (I, [P, D, D, D]),
+    "ss_cyl_timesteps2": (I, [P, D]),
+    "ss_cyl_compute_P": (I, [P, D, D, D]),
+    "ss_cyl_compute_sigma": (I, [P]),
+    "ss_cyl_compute_TII": (I, [P]),
+    "ss_cyl_power_law": (I, [P, D, D, D]),
+    "ss_cyl_compute_dV": (I, [P, D, D, D]),
+    "ss_cyl_compute_V": (I, [P]),
+    "ss_cyl_bc": (I, [P, I, I, D, D]),
+    "ss_cyl_err_rth": (I, [P]),
+    "ss_cyl_exit_error": (I, [P, DP, DP]),
+    "ss_cyl_pt_solve": (I, [P, C.POINTER(ss_cyl_params), I, I, D, C.POINTER(ss_cyl_result)]),
+    "ss_cyl_pt_iterate_timed": (I, [P, C.POINTER(ss_cyl_params), I, DP, DP]),
 })
 
 _LIB = None

@@ -26,7 +26,7 @@ EMU_SRC = [ROOT / "tests" / "emu" / "fused_emu.cpp", ROOT / "tests" / "emu" / "a
 EMU_LIB = ROOT / "tests" / "emu" / "libsspt_emu.so"
 
 HIP_SOURCES = ["sspt_api.hip", "sspt_kernels.hip", "sspt_fused.hip", "sspt_mixed.hip", "sspt_adjoint.hip",
-               "sspt_noalloc.hip", "sspt_cart.hip"]
+               "sspt_noalloc.hip", "sspt_cart.hip", "sspt_cyl.hip"]
 ARCH = os.environ.get("SSPT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

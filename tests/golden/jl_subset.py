@@ -3,8 +3,10 @@ written in, used to produce golden vectors from the reference's own kernel
 text (test infrastructure; nothing in the product imports it).
 
 It reads a reference script as text, collects its expression macros
-(``macro NAME(ix,iy,iz) esc(:( ... )) end``) and its ``@parallel_indices
-(ix,iy,iz) function NAME(...) ... end`` kernels, expands the macros the way
+(``macro NAME(ix,iy,iz) esc(:( ... )) end``), its ``@parallel_indices
+(ix,iy,iz) function NAME(...) ... end`` kernels and its ``@parallel function
+NAME(...) ... end`` kernels (whole-array statements with ParallelStencil's
+FiniteDifferences3D macros, restated below as FD3D_TEXT), expands the macros the way
 Julia does (argument expressions substituted for ``$param``, every expansion
 one sub-expression) and translates each kernel body into a Python function
 that runs point by point over the launch ranges.  Floating-point semantics
@@ -203,7 +205,11 @@ def expand(toks: Sequence[Tok], macros: Dict[str, Macro], depth: int = 0) -> Lis
             body: List[Tok] = []
             for b in m.body:
                 if b.kind == "interp" and b.text in sub:
-                    body += [Tok("op", "(")] + list(sub[b.text]) + [Tok("op", ")")]
+                    arg = list(sub[b.text])
+                    if len(arg) == 1 and arg[0].kind in ("id", "num"):
+                        body += arg             # a name stays a name (A[...] indexing)
+                    else:
+                        body += [Tok("op", "(")] + arg + [Tok("op", ")")]
                 else:
                     body.append(b)
             out += [Tok("op", "(")] + expand(body, macros, depth + 1) + [Tok("op", ")")]
@@ -224,8 +230,83 @@ class Kernel:
     indices: List[str]
     params: List[str]
     body: List[Tok]
+    kind: str = "indices"   # "indices" (@parallel_indices) or "parallel" (@parallel)
     pysrc: str = ""
     fn: object = None
+
+
+# ParallelStencil.FiniteDifferences3D's macros for @parallel kernels,
+# restated from the package's published definitions (the reference scripts
+# load them, CylindricalStokes_diapir.jl:7-8; the package is not part of the
+# reference and its version is unpinned there): differences and averages
+# over "a"ll points or the "i"nner points (index + 1) of the other
+# directions, and the 7-point @maxloc nesting that
+# SphericalStokes_mxpu.jl:295-297 writes out.
+FD3D_TEXT = """
+macro all(A)    esc(:( $A[ix  ,iy  ,iz  ] )) end
+macro inn(A)    esc(:( $A[ix+1,iy+1,iz+1] )) end
+macro d_xa(A)   esc(:( $A[ix+1,iy  ,iz  ] - $A[ix  ,iy  ,iz  ] )) end
+macro d_ya(A)   esc(:( $A[ix  ,iy+1,iz  ] - $A[ix  ,iy  ,iz  ] )) end
+macro d_za(A)   esc(:( $A[ix  ,iy  ,iz+1] - $A[ix  ,iy  ,iz  ] )) end
+macro d_xi(A)   esc(:( $A[ix+1,iy+1,iz+1] - $A[ix  ,iy+1,iz+1] )) end
+macro d_yi(A)   esc(:( $A[ix+1,iy+1,iz+1] - $A[ix+1,iy  ,iz+1] )) end
+macro d_zi(A)   esc(:( $A[ix+1,iy+1,iz+1] - $A[ix+1,iy+1,iz  ] )) end
+macro av_xa(A)  esc(:( ($A[ix  ,iy  ,iz  ] + $A[ix+1,iy  ,iz  ])*0.5 )) end
+macro av_ya(A)  esc(:( ($A[ix  ,iy  ,iz  ] + $A[ix  ,iy+1,iz  ])*0.5 )) end
+macro av_za(A)  esc(:( ($A[ix  ,iy  ,iz  ] + $A[ix  ,iy  ,iz+1])*0.5 )) end
+macro av_xi(A)  esc(:( ($A[ix  ,iy+1,iz+1] + $A[ix+1,iy+1,iz+1])*0.5 )) end
+macro av_yi(A)  esc(:( ($A[ix+1,iy  ,iz+1] + $A[ix+1,iy+1,iz+1])*0.5 )) end
+macro av_zi(A)  esc(:( ($A[ix+1,iy+1,iz  ] + $A[ix+1,iy+1,iz+1])*0.5 )) end
+macro av_xya(A) esc(:( ($A[ix,iy,iz] + $A[ix+1,iy,iz] + $A[ix,iy+1,iz] + $A[ix+1,iy+1,iz])*0.25 )) end
+macro av_xza(A) esc(:( ($A[ix,iy,iz] + $A[ix+1,iy,iz] + $A[ix,iy,iz+1] + $A[ix+1,iy,iz+1])*0.25 )) end
+macro av_yza(A) esc(:( ($A[ix,iy,iz] + $A[ix,iy+1,iz] + $A[ix,iy,iz+1] + $A[ix,iy+1,iz+1])*0.25 )) end
+macro av_xyi(A) esc(:( ($A[ix,iy,iz+1] + $A[ix+1,iy,iz+1] + $A[ix,iy+1,iz+1] + $A[ix+1,iy+1,iz+1])*0.25 )) end
+macro av_xzi(A) esc(:( ($A[ix,iy+1,iz] + $A[ix+1,iy+1,iz] + $A[ix,iy+1,iz+1] + $A[ix+1,iy+1,iz+1])*0.25 )) end
+macro av_yzi(A) esc(:( ($A[ix+1,iy,iz] + $A[ix+1,iy+1,iz] + $A[ix+1,iy,iz+1] + $A[ix+1,iy+1,iz+1])*0.25 )) end
+macro maxloc(A) esc(:( max( max( max( max($A[ix,iy+1,iz+1], $A[ix+2,iy+1,iz+1]), $A[ix+1,iy+1,iz+1]),
+                                      max($A[ix+1,iy,iz+1], $A[ix+1,iy+2,iz+1]) ),
+                                      max($A[ix+1,iy+1,iz], $A[ix+1,iy+1,iz+2]) ) )) end
+"""
+
+
+def _parallel_statements(toks: List[Tok], out: List[str]):
+    """Statements of a @parallel kernel: `@all(A) = rhs` or `@inn(A) = rhs`,
+    each guarded by A's extent (ParallelStencil assigns only where the
+    left-hand array is defined: all points, or the inner points)."""
+    stmts, cur, depth = [], [], 0
+    for i, x in enumerate(toks):
+        if x.kind == "op" and x.text in "([":
+            depth += 1
+        elif x.kind == "op" and x.text in ")]":
+            depth -= 1
+        if x.kind == "nl" and depth == 0:
+            prev = next((t for t in reversed(cur) if t.kind != "nl"), None)
+            if prev is not None and not (prev.kind == "op" and prev.text in _BINOPS):
+                stmts.append(cur)
+                cur = []
+            continue
+        cur.append(x)
+    if cur:
+        stmts.append(cur)
+    for st in stmts:
+        st = [x for x in st if x.kind != "nl"]
+        if not st or (st[0].kind == "id" and st[0].text == "return"):
+            continue
+        if not (st[0].kind == "macro" and st[0].text in ("all", "inn") and st[1].text == "("):
+            raise ValueError("unsupported @parallel statement: " + " ".join(x.text for x in st))
+        close = _match(st, 1)
+        arr = _pyname(st[2].text)
+        if st[close + 1].text != "=":
+            raise ValueError("expected = after the left-hand side")
+        rhs = expr_to_py(expand_tokens(st[close + 2:]))
+        if st[0].text == "all":
+            cond = f"ix <= _size({arr}, 1) and iy <= _size({arr}, 2) and iz <= _size({arr}, 3)"
+            lhs = f"{arr}[ix, iy, iz]"
+        else:
+            cond = f"ix <= _size({arr}, 1) - 2 and iy <= _size({arr}, 2) - 2 and iz <= _size({arr}, 3) - 2"
+            lhs = f"{arr}[ix + 1, iy + 1, iz + 1]"
+        out.append(f"    if {cond}:")
+        out.append(f"        {lhs} = {rhs}")
 
 
 _BINOPS = {"+", "-", "*", "/", "^", "=", "==", "!=", "<=", ">=", "<", ">", "&&", "||", ","}
@@ -256,6 +337,25 @@ def collect_kernels(toks: Sequence[Tok]) -> Dict[str, Kernel]:
                     depth -= 1
                 k += 1
             kernels[name] = Kernel(name, idx, params, list(toks[start:k - 1]))
+            i = k
+            continue
+        if t.kind == "macro" and t.text == "parallel" and i + 1 < len(toks) and toks[i + 1].text == "function":
+            # @parallel function NAME(params) BODY end: whole-array statements
+            # (@all(A) = ..., @inn(A) = ...) over the indices ix, iy, iz
+            j = i + 1
+            name = toks[j + 1].text
+            pc = _match(toks, j + 2)
+            params = [a[0].text for a in _split_args(toks[j + 3:pc])]
+            k, depth = pc + 1, 1
+            start = k
+            while depth:
+                x = toks[k]
+                if x.kind == "id" and x.text in ("if", "for", "while", "function", "begin", "let"):
+                    depth += 1
+                elif x.kind == "id" and x.text == "end":
+                    depth -= 1
+                k += 1
+            kernels[name] = Kernel(name, ["ix", "iy", "iz"], params, list(toks[start:k - 1]), kind="parallel")
             i = k
             continue
         i += 1
@@ -545,14 +645,18 @@ class Script:
     def __init__(self, text: str):
         global _CURRENT_MACROS
         self.toks = tokenize(text)
-        self.macros = collect_macros(self.toks)
+        self.macros = collect_macros(tokenize(FD3D_TEXT))
+        self.macros.update(collect_macros(self.toks))
         self.kernels = collect_kernels(self.toks)
         _CURRENT_MACROS = self.macros
         self.errors: Dict[str, str] = {}
         for k in list(self.kernels.values()):
             lines: List[str] = []
             try:
-                _statements(k.body, 0, False, lines, "    ")
+                if k.kind == "parallel":
+                    _parallel_statements(k.body, lines)
+                else:
+                    _statements(k.body, 0, False, lines, "    ")
             except (ValueError, IndexError) as e:
                 # outside the subset (kernels this evaluator is not used on)
                 self.errors[k.name] = str(e)

@@ -28,6 +28,11 @@ outputs):
   * init: initialize_inclusion! / initialize_velocity! (diapir, 15^3 sector
     grid, squared-distance test with radius 0.01) and initialize_geometry!
     (plateau, on the script's scaled 70 x 5 x 5 grid and on 31 x 19 x 23);
+  * cyl_*: the cylindrical twin, scripts_diapir/CylindricalStokes_diapir.jl,
+    whose kernels are ParallelStencil @parallel kernels: every kernel one at
+    a time, three iterations of its loop body (:332-362) and its setup
+    kernels, with FiniteDifferences3D's macros as jl_subset.FD3D_TEXT
+    restates them;
   * cart_*: the Cartesian twin, scripts_diapir/CartesianStokes_diapir.jl --
     every kernel one at a time (timesteps!, compute_P!, compute_TII!,
     power_law! with relaxation 0.5, compute_V!, copy_boundary!, the
@@ -283,6 +288,152 @@ def cart_vectors(cart, arrays, meta):
     arrays["cart_init__x"] = xs
 
 
+CYL_SHAPES = {  # CylindricalStokes_diapir.jl:278-318
+    "P": (0, 0, 0), "DIVV": (0, 0, 0), "VR": (1, 0, 0), "Vθ": (0, 1, 0), "VZ": (0, 0, 1),
+    "dVR": (-1, -2, -2), "dVθ": (-2, -1, -2), "dVZ": (-2, -2, -1),
+    "VR_OLD": (1, 0, 0), "Vθ_OLD": (0, 1, 0), "VZ_OLD": (0, 0, 1),
+    "D_RR": (0, 0, 0), "D_θθ": (0, 0, 0), "D_ZZ": (0, 0, 0),
+    "D_Rθ": (-1, -1, -2), "D_RZ": (-1, -2, -1), "D_θZ": (-2, -1, -1),
+    "τ_RR": (0, 0, 0), "τ_θθ": (0, 0, 0), "τ_ZZ": (0, 0, 0),
+    "τ_Rθ": (-1, -1, -2), "τ_RZ": (-1, -2, -1), "τ_θZ": (-2, -1, -1),
+    "σ_RR": (0, 0, 0), "σ_θθ": (0, 0, 0), "σ_ZZ": (0, 0, 0),
+    "ρ": (0, 0, 0), "ρG": (0, 0, 0), "η": (0, 0, 0), "η_INI": (0, 0, 0), "η_PL": (0, 0, 0),
+    "η_PL_OLD": (0, 0, 0), "ηSM": (0, 0, 0), "KBDT": (0, 0, 0), "GSDT": (0, 0, 0), "DT_R": (0, 0, 0),
+    "τII": (0, 0, 0), "ERR_Rθ": (-1, -1, -2),
+}
+
+
+def cyl_ascii(name):
+    return (name.replace("τ_", "tau_").replace("σ_", "sig_").replace("θ", "th").replace("ηSM", "etaSM")
+            .replace("η", "eta").replace("ρG", "rhoG").replace("ρ", "rho").replace("τII", "tII"))
+
+
+def cyl_vectors(cyl, arrays, meta):
+    """Kernels of CylindricalStokes_diapir.jl one at a time on a seeded
+    random state (7 x 6 x 8 cells of a wide sector r in [1, 2], th in
+    [-pi/6, pi/6], z in [-0.5, 0.5]), three iterations of its loop body
+    (:332-362) with n_exp = 3 and the exit-test quantities on the third, and
+    its setup kernels."""
+    nr, nt, nz = 7, 6, 8
+    r, lr, lt, lz = 1.5, 1.0, math.pi / 3.0, 1.0
+    R1 = np.linspace(r - lr / 2.0, r + lr / 2.0, nr)
+    t1 = np.linspace(-lt / 2.0, lt / 2.0, nt)
+    z1 = np.linspace(-lz / 2.0, lz / 2.0, nz)
+    dr, dt, dz = lr / (nr - 1), lt / (nt - 1), lz / (nz - 1)
+    CFL = 1.0 / (2.0 + 4.5 * math.log10(1e2))
+    sc = dict(dr=dr, dθ=dt, dz=dz, dmp=4.5, vpdt=dr * CFL, lr=lr, τ_C=1.0, n_exp=3.0, relax=0.5, g=1.0,
+              sh=1.0, r=r)
+    g = np.random.default_rng(31)
+    st0 = {}
+    for name, d in CYL_SHAPES.items():
+        shp = (nr + d[0], nt + d[1], nz + d[2])
+        if name in ("η", "η_INI", "η_PL", "η_PL_OLD", "ηSM", "KBDT", "GSDT", "DT_R", "τII"):
+            a = 10.0 ** g.normal(0.0, 0.5, shp)
+        else:
+            a = g.normal(0.0, 1.0, shp)
+        st0[name] = np.asfortranarray(a)
+    R3 = np.asfortranarray(np.broadcast_to(R1[:, None, None], (nr, nt, nz)).copy())
+    coords = {"R": R3,
+              "θ_VZ": np.asfortranarray(np.broadcast_to(t1[None, :, None], (nr, nt, nz + 1)).copy()),
+              "θ_VR": np.asfortranarray(np.broadcast_to(t1[None, :, None], (nr + 1, nt, nz)).copy())}
+    meta["cyl"] = {"grid": [nr, nt, nz], "scalars": {cyl_ascii(k).replace("dth", "dth"): v for k, v in sc.items()}}
+    arrays["cyl__R"], arrays["cyl__th"], arrays["cyl__z"] = R1, t1, z1
+    for n, a in st0.items():
+        arrays[f"cyl_in__state__{cyl_ascii(n)}"] = a
+
+    def put(case, name, a):
+        arrays[f"cyl_{case}__out__{cyl_ascii(name)}"] = np.asarray(a)
+
+    def run(st, kname, **over):
+        k = cyl.kernels[kname]
+        env = {**sc, **coords, **st, **over}
+        cyl.run(kname, *[env[p] for p in k.params])
+
+    one = {"timesteps!": ["KBDT", "GSDT"], "timesteps2!": ["DT_R"],
+           "compute_P!": ["D_RR", "D_θθ", "D_ZZ", "D_Rθ", "D_RZ", "D_θZ", "DIVV", "P", "τ_RR", "τ_θθ", "τ_ZZ",
+                          "τ_Rθ", "τ_RZ", "τ_θZ"],
+           "compute_sigma!": ["σ_RR", "σ_θθ", "σ_ZZ"], "compute_TII!": ["τII"], "power_law!": ["η_PL", "η"],
+           "compute_dV!": ["dVR", "dVθ", "dVZ"], "compute_V!": ["VR", "Vθ", "VZ"], "err_rθ!": ["ERR_Rθ"]}
+    for kname, outs in one.items():
+        st = copy(st0)
+        over = {"η_M": st["ηSM"]} if kname == "timesteps!" else ({"err_rθ": st["ERR_Rθ"]} if kname == "err_rθ!" else {})
+        run(st, kname, **over)
+        for n in outs:
+            assert not np.array_equal(st[n], st0[n]), (kname, n)
+            put(kname.rstrip("!").replace("θ", "th"), n, st[n])
+    st = copy(st0)
+    run(st, "maxloc!", η_Max=st["ηSM"])
+    put("maxloc", "ηSM", st["ηSM"])
+    bcs = [("copy_bc_r!", "VZ", {}), ("copy_bc_θ!", "ηSM", {}), ("copy_bc_z!", "τII", {}), ("bc_r_0!", "VR", {}),
+           ("bc_θ_fact!", "VZ", {"COORD": "θ_VZ", "fact": 1.0}), ("bc_θ_fact!", "VR", {"COORD": "θ_VR", "fact": 0.0}),
+           ("bc_θ_0!", "Vθ", {}), ("bc_z_0!", "VR", {}), ("bc_z_lin!", "VZ", {"COORD": "θ_VZ", "fact": 1.0})]
+    for kname, fld, extra in bcs:
+        st = copy(st0)
+        over = {"A": st[fld]}
+        if "COORD" in extra:
+            over["COORD"] = coords[extra["COORD"]]
+            over["fact"] = extra["fact"]
+        run(st, kname, **over)
+        put(f"{kname.rstrip('!').replace('θ', 'th')}_{cyl_ascii(fld)}", fld, st[fld])
+
+    # three iterations of the loop body (err < 1e-5 never holds here, so
+    # n_exp stays 3), the exit-test copies and err_rth! on the third
+    st = copy(st0)
+    for it in (1, 2, 3):
+        run(st, "equal!", A=st["ηSM"], B=st["η"])
+        run(st, "maxloc!", η_Max=st["ηSM"])
+        for c in ("copy_bc_r!", "copy_bc_θ!", "copy_bc_z!"):
+            run(st, c, A=st["ηSM"])
+        run(st, "timesteps!", η_M=st["ηSM"])
+        run(st, "timesteps2!")
+        run(st, "compute_P!")
+        run(st, "compute_sigma!")
+        run(st, "compute_TII!")
+        for c in ("copy_bc_r!", "copy_bc_θ!", "copy_bc_z!"):
+            run(st, c, A=st["τII"])
+        run(st, "equal!", A=st["η_PL_OLD"], B=st["η_PL"])
+        run(st, "power_law!")
+        run(st, "equal!", A=st["VR_OLD"], B=st["VR"])
+        run(st, "equal!", A=st["Vθ_OLD"], B=st["Vθ"])
+        run(st, "equal!", A=st["VZ_OLD"], B=st["VZ"])
+        run(st, "compute_dV!")
+        run(st, "compute_V!")
+        run(st, "copy_bc_r!", A=st["VZ"])
+        run(st, "copy_bc_r!", A=st["Vθ"])
+        run(st, "bc_r_0!", A=st["VR"])
+        run(st, "bc_θ_fact!", A=st["VZ"], COORD=coords["θ_VZ"], fact=sc["sh"])
+        run(st, "bc_θ_fact!", A=st["VR"], COORD=coords["θ_VR"], fact=0.0)
+        run(st, "bc_θ_0!", A=st["Vθ"])
+        run(st, "bc_z_0!", A=st["Vθ"])
+        run(st, "bc_z_0!", A=st["VR"])
+        run(st, "bc_z_lin!", A=st["VZ"], COORD=coords["θ_VZ"], fact=sc["sh"])
+    run(st, "err_rθ!", err_rθ=st["ERR_Rθ"])
+    for n in CYL_SHAPES:
+        put("loop", n, st[n])
+    # the exit test's error of the third iteration (host code :364-367):
+    # maximum(abs.(V .- V_OLD)) ./ maximum(abs.(V)) per component
+    arrays["cyl_loop__err"] = np.array([float(np.max(np.abs(st[v] - st[v + "_OLD"]))) / float(np.max(np.abs(st[v])))
+                                        for v in ("VR", "Vθ", "VZ")])
+
+    # setup kernels, 15^3 thin sector as the script (r = 1000, lr = lz = 6, lth = lr/r)
+    n = 15
+    Rs = np.linspace(1000.0 - 3.0, 1000.0 + 3.0, n)
+    ts = np.linspace(-0.003, 0.003, n)
+    zs = np.linspace(-3.0, 3.0, n)
+    A = np.asfortranarray(np.ones((n, n, n)))
+    cyl.run("initialize_inclusion!", A,
+            np.asfortranarray(np.broadcast_to(Rs[:, None, None], (n, n, n)).copy()),
+            np.asfortranarray(np.broadcast_to(ts[None, :, None], (n, n, n)).copy()),
+            np.asfortranarray(np.broadcast_to(zs[None, None, :], (n, n, n)).copy()), 1000.0, 1.0, 0.01)
+    assert 0 < np.count_nonzero(A != 1.0) < A.size
+    arrays["cyl_init__out__inclusion"] = A
+    V = np.asfortranarray(np.zeros((n, n, n + 1)))
+    cyl.run("initialize_velocity!", V, np.asfortranarray(np.broadcast_to(ts[None, :, None], (n, n, n + 1)).copy()),
+            1.0, 1000.0)
+    arrays["cyl_init__out__velocity"] = V
+    arrays["cyl_init__R"], arrays["cyl_init__th"], arrays["cyl_init__z"] = Rs, ts, zs
+
+
 def _maxloc_inner(eta):
     """@maxloc(eta) on the inner points (ParallelStencil's nesting, the
     expansion written out in SphericalStokes_mxpu.jl:295-297); Julia's max
@@ -397,6 +548,9 @@ def main():
 
     # ---- the Cartesian twin (CartesianStokes_diapir.jl)
     cart_vectors(cart, arrays, meta)
+    # ---- the cylindrical twin (CylindricalStokes_diapir.jl)
+    cyl = J.Script((REF / "scripts_diapir" / "CylindricalStokes_diapir.jl").read_text())
+    cyl_vectors(cyl, arrays, meta)
 
     np.savez_compressed(OUT.with_suffix(".npz"), **arrays)
     OUT.with_suffix(".json").write_text(json.dumps(meta, indent=1, sort_keys=True) + "\n")

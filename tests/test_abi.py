@@ -43,6 +43,11 @@ def test_create_rejects_bad_sizes_without_gpu():
     assert L.ss_cart_last_error(None) == b"null context"
     L.ss_cart_destroy(None)
     assert L.ss_cart_maxloc(None) == _lib.SS_ERR_ARG
+    assert L.ss_cyl_create(3, 10, 10, 0, C.byref(h)) == _lib.SS_ERR_ARG
+    assert L.ss_cyl_create(10, 10, 10, 0, None) == _lib.SS_ERR_ARG
+    assert L.ss_cyl_last_error(None) == b"null context"
+    L.ss_cyl_destroy(None)
+    assert L.ss_cyl_compute_V(None) == _lib.SS_ERR_ARG
 
 
 def test_params_struct_layout():
@@ -51,6 +56,8 @@ def test_params_struct_layout():
     assert C.sizeof(_lib.ss_result) == 4 + 4 + 8 + 24 + 8
     assert C.sizeof(_lib.ss_cart_params) == 11 * 8
     assert C.sizeof(_lib.ss_cart_result) == 4 + 4 + 8 + 8
+    assert C.sizeof(_lib.ss_cyl_params) == 13 * 8
+    assert C.sizeof(_lib.ss_cyl_result) == 4 + 4 + 8 + 8 + 8
 
 
 def test_check_err_noalloc_rejects_without_gpu():

@@ -1,6 +1,6 @@
 """GPU: bench.py end to end on a small grid, in a child process: it prints one
 JSON line with the fields the measurement contract needs (metric, value,
-roofline, cpu_baseline, the mixed-precision, adjoint and Cartesian-twin legs, and the
+roofline, cpu_baseline, the mixed-precision, adjoint and twin legs, and the
 streaming rates of the torch probe, which runs in a grandchild process
 because torch's HIP runtime cannot share the GPU with libsspt.so's)."""
 import json
@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 def test_bench_json_contract():
     cmd = [sys.executable, str(ROOT / "bench.py"), "--n", "63", "--steps", "3", "--warmup", "1", "--cpu-iters", "1",
-           "--ttt-n", "0", "--adjoint-n", "0", "--fp32-sweeps", "2", "--adjoint-steps", "1", "--cart-n", "31"]
+           "--ttt-n", "0", "--adjoint-n", "0", "--fp32-sweeps", "2", "--adjoint-steps", "1", "--cart-n", "31", "--cyl-n", "31"]
     print("running", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=str(ROOT))
     assert r.returncode == 0, r.stderr[-2000:]
@@ -36,8 +36,9 @@ def test_bench_json_contract():
     assert cb["kind"] == "port" and cb["value"] > 0 and cb["cores"] >= 1 and cb["sample"]
     assert out["mixed_precision"]["fp32_sweep_ms"] > 0
     assert out["adjoint"]["iter_ms"] > 0
-    ct = out["cartesian_twin"]
-    assert ct["iter_ms"] > 0 and ct["T_eff_GBps"] > 0 and 0 < ct["compute_P"]["frac_of_peak"] < 1
+    for twin in ("cartesian_twin", "cylindrical_twin"):
+        ct = out[twin]
+        assert ct["iter_ms"] > 0 and ct["T_eff_GBps"] > 0 and 0 < ct["compute_P"]["frac_of_peak"] < 1, twin
     hb = out["hbm_measured"]
     assert "error" not in hb, hb
     assert all(hb[k] > 0 for k in ("copy_GBps", "triad_GBps", "read_GBps", "write_GBps"))
