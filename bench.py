@@ -255,7 +255,7 @@ def cylindrical_twin(n, steps, warmup=3):
     thin shell r = 1000, linear phase n_exp = 1) at n^3: one iteration is the
     script's kernel sequence (:331-366, about 25 launches).  T_eff by the
     spherical path's A_eff accounting; the roofline of compute_P!, whose
-    algorithmic bytes per cell are 24 x 8 (reads VR Vth VZ KBDT GSDT eta,
+    algorithmic bytes per cell are 27 x 8 (reads VR Vth VZ KBDT GSDT eta,
     reads and writes P and the six stresses, writes DIVV and the six strain
     rates)."""
     import numpy as np

@@ -30,7 +30,7 @@ outputs):
     (plateau, on the script's scaled 70 x 5 x 5 grid and on 31 x 19 x 23);
   * cyl_*: the cylindrical twin, scripts_diapir/CylindricalStokes_diapir.jl,
     whose kernels are ParallelStencil @parallel kernels: every kernel one at
-    a time, three iterations of its loop body (:332-362) and its setup
+    a time, three iterations of its loop body (:331-366) and its setup
     kernels, with FiniteDifferences3D's macros as jl_subset.FD3D_TEXT
     restates them;
   * cart_*: the Cartesian twin, scripts_diapir/CartesianStokes_diapir.jl --
@@ -312,7 +312,7 @@ def cyl_vectors(cyl, arrays, meta):
     """Kernels of CylindricalStokes_diapir.jl one at a time on a seeded
     random state (7 x 6 x 8 cells of a wide sector r in [1, 2], th in
     [-pi/6, pi/6], z in [-0.5, 0.5]), three iterations of its loop body
-    (:332-362) with n_exp = 3 and the exit-test quantities on the third, and
+    (:331-366) with n_exp = 3 and the exit-test quantities on the third, and
     its setup kernels."""
     nr, nt, nz = 7, 6, 8
     r, lr, lt, lz = 1.5, 1.0, math.pi / 3.0, 1.0
