@@ -159,15 +159,34 @@ struct ALds {
     // phase A -> B; QX, QY, GSM, ETA for planes q and q-1 (index = plane & 1)
     double wVR[ATJ][TK], wVT[ATJ][TK], wVP[ATJ][TK], QZ[ATJ][TK];
     double QX[2][ATJ][TK], QY[2][ATJ][TK], GSM[2][ATJ][TK], ETA[2][ATJ][TK];
-    // phase B -> C
-    double tV[ATJ][TK], dP[ATJ][TK]; // centre terms of Vth at j+1 and of Vph at k+1
-    double gRT[ATJ][TK], gRP[ATJ][TK], gTP[ATJ][TK], cRT[ATJ][TK], cRP[ATJ][TK], cTP[ATJ][TK];
+    // DT_R of planes q and q+1 (index = plane & 1), written one step ahead:
+    // the j-1 / k-1 neighbours' values for the face weights of Vth / Vph
+    double DTL[2][ATJ][TK];
+    // phase B -> C (values read only from the lane next door, gRP, cRP and
+    // the centre term of Vph, come by lane shuffles instead)
+    double tV[ATJ][TK]; // centre term of Vth at j+1
+    double gRT[ATJ][TK], gTP[ATJ][TK], cRT[ATJ][TK], cTP[ATJ][TK];
     // each thread's own accumulators (Vth, Vph, eta of plane q-1; VR(q)'s
     // term of centre q-1) and w of its th/phi faces of plane q-1: kept here
     // rather than in registers
     double ACC[6][ATJ][TK];
 };
 enum { ACC_VT, ACC_VP, ACC_E, ACC_VRN, ACC_WVT, ACC_WVP };
+
+// Member m of the thread d = +1 / -1 lanes away in the same row when
+// `in_row` (the neighbour lies in the row), else 0.  On the device a row is
+// one wavefront and the value comes by a lane shuffle that every lane
+// executes; the host emulation keeps a row's threads contiguous in memory
+// and reads the neighbour's state.
+template <typename S> SSPT_DH inline double lane_nb(const S &self, double S::*m, int d, bool in_row)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    const double v = d > 0 ? __shfl_down(self.*m, 1, TK) : __shfl_up(self.*m, 1, TK);
+    return in_row ? v : 0.0;
+#else
+    return in_row ? (&self)[d].*m : 0.0;
+#endif
+}
 
 struct AThread {
     int tx, ty, j, k;
@@ -197,13 +216,15 @@ SSPT_DH inline void athread_init(AThread &T, const AArgs &a, int tx, int ty, int
     T.DTq = T.wVRq = T.lVRq = 0.0;
 }
 
-// DT(q), lam_VR(q) and w_VR(q) of the first plane of the march
+// DT(q), lam_VR(q) and w_VR(q) of the first plane of the march (DT(q) also
+// into the LDS plane that the first phase A reads: a barrier follows)
 SSPT_DH inline void aprologue(AThread &T, ALds &L, const AArgs &a, int q)
 {
     for (int m = 0; m < 6; ++m) L.ACC[m][T.ty][T.tx] = 0.0;
     const BoxG &b = a.b;
     const int qc = clampi(q, 0, b.nr);
     T.DTq = fat(a.DT, b.o32(qc, T.jc, T.kc));
+    L.DTL[q & 1][T.ty][T.tx] = T.DTq;
     T.lVRq = fat(a.li[A_VR], b.o32(qc, T.jc, T.kc));
     const double DTm = fat(a.DT, b.o32(clampi(q - 1, 0, b.nr), T.jc, T.kc));
     T.wVRq = upd_VR(b, q, T.j, T.k) ? ((DTm + T.DTq) * 0.5) * T.lVRq : 0.0;
@@ -219,10 +240,13 @@ SSPT_DH inline void aprologue(AThread &T, ALds &L, const AArgs &a, int q)
 // phase B (loaded at the start of phase A).  Each half's latency hides
 // behind a phase and a barrier, with one half in flight at a time.
 struct AStepA {
-    double DT1, lVR1, lVT, lVP, DTjm, DTkm;
+    double DT1, lVR1, lVT, lVP;
+    double DTjm, DTkm;               // DT(q) at j-1 (first row) / k-1 (first lane); the others read LDS
     double le, tii, ini, ePL, lPL, lb, sm, e;
     double RT1, RT1j, RP1, RP1k;     // tau_Rth / tau_Rph of box index q+1 at (j, j+1) / (k, k+1)
     double TP, TPk, TPj, TPjk;       // tau_thph of box index q at (j,k), (j,k+1), (j+1,k), (j+1,k+1)
+    // (the k+1 values RP1k, TPk, TPjk are loaded by the last lane only; the
+    // others take them from the lane next door)
 };
 struct AStepB {
     double tRR, tTT, tPP;
@@ -241,8 +265,8 @@ SSPT_DH inline void aloadA(AStepA &S, const AArgs &a, const AThread &T, int q)
     S.lVR1 = fat(a.li[A_VR], o1);
     S.lVT = fat(a.li[A_VT], o);
     S.lVP = fat(a.li[A_VP], o);
-    S.DTjm = fat(a.DT, b.o32(qc, clampi(T.jc - 1, 0, b.nt), T.kc));
-    S.DTkm = fat(a.DT, b.o32(qc, T.jc, clampi(T.kc - 1, 0, b.np)));
+    S.DTjm = T.ty == 0 ? fat(a.DT, b.o32(qc, clampi(T.jc - 1, 0, b.nt), T.kc)) : 0.0;
+    S.DTkm = T.tx == 0 ? fat(a.DT, b.o32(qc, T.jc, clampi(T.kc - 1, 0, b.np))) : 0.0;
     S.le = fat(a.li[A_E], o);
     S.tii = fat(a.tII, o);
     S.ini = fat(a.ini, o);
@@ -252,11 +276,12 @@ SSPT_DH inline void aloadA(AStepA &S, const AArgs &a, const AThread &T, int q)
     S.RT1 = fat(a.tRT, o1);
     S.RT1j = fat(a.tRT, b.o32(clampi(q + 1, 0, b.nr), jc1, T.kc));
     S.RP1 = fat(a.tRP, o1);
-    S.RP1k = fat(a.tRP, b.o32(clampi(q + 1, 0, b.nr), T.jc, kc1));
+    const bool lastl = T.tx == TK - 1;
+    S.RP1k = lastl ? fat(a.tRP, b.o32(clampi(q + 1, 0, b.nr), T.jc, kc1)) : 0.0;
     S.TP = fat(a.tTP, o);
-    S.TPk = fat(a.tTP, b.o32(qc, T.jc, kc1));
+    S.TPk = lastl ? fat(a.tTP, b.o32(qc, T.jc, kc1)) : 0.0;
     S.TPj = fat(a.tTP, b.o32(qc, jc1, T.kc));
-    S.TPjk = fat(a.tTP, b.o32(qc, jc1, kc1));
+    S.TPjk = lastl ? fat(a.tTP, b.o32(qc, jc1, kc1)) : 0.0;
     // LB is read only next to the boundary (the condition depends on the
     // position alone, so the load still issues with the others)
     S.lb = next_to_boundary(b, q, T.j, T.k) ? fat(a.LB, o) : 0.0;
@@ -295,20 +320,34 @@ SSPT_DH inline void aphaseA(AThread &T, ALds &L, const AArgs &a, const AStepA &S
     T.lVP = S.lVP;
     const double le = S.le, tii = S.tii, ini = S.ini, ePL = S.ePL, lPL = S.lPL;
     const double lb = S.lb, sm = S.sm;
+    // the k+1 neighbours' shear stresses: lane tx+1 loaded them as its own
+    // (where they are used, at inner cells, k+1 is inside the box, so the
+    // clamped positions agree); the last lane loaded its own
+    const bool nk = T.tx + 1 < TK;
+    const double nRP1 = lane_nb(S, &AStepA::RP1, +1, nk), nTP = lane_nb(S, &AStepA::TP, +1, nk);
+    const double nTPj = lane_nb(S, &AStepA::TPj, +1, nk);
+    const double RP1k = nk ? nRP1 : S.RP1k, TPk = nk ? nTP : S.TPk, TPjk = nk ? nTPj : S.TPjk;
     // compute_TII!'s 4-point shear averages (times 1/4) of the cell
     const double sX = 0.25 * ((((T.RTq + T.RTqj) + S.RT1) + S.RT1j) * 0.25);
-    const double sY = 0.25 * ((((T.RPq + T.RPqk) + S.RP1) + S.RP1k) * 0.25);
-    const double sZ = 0.25 * ((((S.TP + S.TPk) + S.TPj) + S.TPjk) * 0.25);
+    const double sY = 0.25 * ((((T.RPq + T.RPqk) + S.RP1) + RP1k) * 0.25);
+    const double sZ = 0.25 * ((((S.TP + TPk) + S.TPj) + TPjk) * 0.25);
     T.tRT = T.RTq;
     T.tRP = T.RPq;
     T.tTP = S.TP;
     T.RTn = S.RT1;
     T.RTnj = S.RT1j;
     T.RPn = S.RP1;
-    T.RPnk = S.RP1k;
+    T.RPnk = RP1k;
     T.e = S.e;
-    T.wVT = upd_VT(b, q, j, k) ? ((S.DTjm + T.DTq) * 0.5) * T.lVT : 0.0;
-    T.wVP = upd_VP(b, q, j, k) ? ((S.DTkm + T.DTq) * 0.5) * T.lVP : 0.0;
+    // DT(q) of the j-1 / k-1 neighbours, written by them one step ahead (where
+    // the weights are used, j-1 and k-1 are inside the box, so the clamped
+    // positions of neighbour and thread agree); DT(q+1) for the next step
+    const int c = q & 1;
+    const double DTjm = T.ty >= 1 ? L.DTL[c][T.ty - 1][T.tx] : S.DTjm;
+    const double DTkm = T.tx >= 1 ? L.DTL[c][T.ty][T.tx - 1] : S.DTkm;
+    L.DTL[c ^ 1][T.ty][T.tx] = T.DTq1;
+    T.wVT = upd_VT(b, q, j, k) ? ((DTjm + T.DTq) * 0.5) * T.lVT : 0.0;
+    T.wVP = upd_VP(b, q, j, k) ? ((DTkm + T.DTq) * 0.5) * T.lVP : 0.0;
     // power-law adjoint and Q of the cell
     double Q = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
     if (is_cell(b, q, j, k)) {
@@ -330,7 +369,6 @@ SSPT_DH inline void aphaseA(AThread &T, ALds &L, const AArgs &a, const AStepA &S
     L.wVR[T.ty][T.tx] = T.wVRq;
     L.wVT[T.ty][T.tx] = T.wVT;
     L.wVP[T.ty][T.tx] = T.wVP;
-    const int c = q & 1;
     L.QX[c][T.ty][T.tx] = qx;
     L.QY[c][T.ty][T.tx] = qy;
     L.QZ[T.ty][T.tx] = qz;
@@ -454,12 +492,9 @@ SSPT_DH inline void aphaseB(AThread &T, ALds &L, const AArgs &a, const AStepB &S
         T.cTP = (0.25 * T.tTP * (g4 / (e4 * e4))) * v;
     }
     L.tV[ty][tx] = T.cw_dT * (a._dth * tload(t.tr, qt * NTR + fused::TR_invRc)) + 0.5 * T.cw_aT;
-    L.dP[ty][tx] = T.cw_dP;
     L.gRT[ty][tx] = T.gRT;
-    L.gRP[ty][tx] = T.gRP;
     L.gTP[ty][tx] = T.gTP;
     L.cRT[ty][tx] = T.cRT;
-    L.cRP[ty][tx] = T.cRP;
     L.cTP[ty][tx] = T.cTP;
 }
 
@@ -473,9 +508,11 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
     const int j = T.j, k = T.k, tx = T.tx, ty = T.ty;
     const int qt = clampi(q, 0, nr), jt = clampi(j, 0, nt), jt1 = clampi(j + 1, 0, nt);
     const bool up = ty >= 1, lf = tx >= 1, dn = ty + 1 < ATJ, rt_ = tx + 1 < TK;
-    const double gRTj = dn ? L.gRT[ty + 1][tx] : 0.0, gRPk = rt_ ? L.gRP[ty][tx + 1] : 0.0;
+    const double gRPk = lane_nb(T, &AThread::gRP, +1, rt_), cRPk = lane_nb(T, &AThread::cRP, +1, rt_);
+    const double dPm = lane_nb(T, &AThread::cw_dP, -1, lf); // centre k-1's term of Vph(k)
+    const double gRTj = dn ? L.gRT[ty + 1][tx] : 0.0;
     const double gTPk = rt_ ? L.gTP[ty][tx + 1] : 0.0, gTPj = dn ? L.gTP[ty + 1][tx] : 0.0;
-    const double cRTj = dn ? L.cRT[ty + 1][tx] : 0.0, cRPk = rt_ ? L.cRP[ty][tx + 1] : 0.0;
+    const double cRTj = dn ? L.cRT[ty + 1][tx] : 0.0;
     const double iRe = tload(t.tr, qt * NTR + fused::TR_invRe);
     const bool inj = j >= 0 && j < nt, ink = k >= 0 && k < np;
     // ---- the finished Vth, Vph, eta of plane q-1 (edges of box index q)
@@ -518,7 +555,7 @@ SSPT_DH inline void aphaseC(AThread &T, ALds &L, const AArgs &a, int q, bool wr,
         if (inj && k >= 0 && k <= np) {
             double v = T.lVP;
             const double rc = tload(t.t2, (qt * (nt + 1) + jt) * NT2 + fused::T2_c) * a._dph;
-            if (lf) v += L.dP[ty][tx - 1] * rc;
+            if (lf) v += dPm * rc;
             v -= T.cw_dP * rc;
             v += T.gRP * pR;
             const double iR4 = tload(t.tr, qt * NTR + fused::TR_invRc4);

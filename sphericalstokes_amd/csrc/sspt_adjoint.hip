@@ -672,6 +672,7 @@ __global__ void __launch_bounds__(fused::TK * adjf::ATJ) k_adj_fused(adjf::AArgs
     aprologue(T, L, a, start);
     AStepA na;
     aloadA(na, a, T, start);
+    __syncthreads(); // the first phase A reads the DT(start) plane the prologue wrote
     for (int q = start; q <= stop; ++q) {
         const bool wr = q >= s0 && q < s1, wr_m = q - 1 >= s0 && q - 1 < s1;
         const AStepA ca = na;

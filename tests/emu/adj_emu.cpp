@@ -166,8 +166,9 @@ int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const doub
                     for (size_t t = 0; t < th.size(); ++t) {
                         ca[t] = na[t];
                         aloadB(cb[t], a, th[t], q);
-                        aphaseA(th[t], *L, a, ca[t], q, wr);
                     }
+                    // phase A reads the loaded values of the lane next door
+                    for (size_t t = 0; t < th.size(); ++t) aphaseA(th[t], *L, a, ca[t], q, wr);
                     for (size_t t = 0; t < th.size(); ++t) {
                         aloadA(na[t], a, th[t], q + 1);
                         aphaseB(th[t], *L, a, cb[t], q, wr);
