@@ -99,9 +99,13 @@ extern "C" {
 // One adjoint step lam_out = T^T lam_in on box-layout buffers (shape
 // (nr+1) x (nth+1) x pitch, phi fastest).  state: tau_RR tau_thth tau_phph
 // tau_Rth tau_Rph tau_thph eta eta_ini eta_PL tII DT_R etaSM; lam_in is
-// BC-folded in place; lam_out must be zero-initialised.
+// BC-folded in place; lam_out must be zero-initialised.  With bJ (box layout,
+// the objective weights: zero outside the inclusion) the step adds bJ on the
+// r-faces that the objective test (centre obj_r, squared radius obj_rad) finds
+// inside.
 int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const double *th1, const double *ph1,
-                 const ss_params *p, double **state, double **lam_in, double **lam_out, int rchunk)
+                 const ss_params *p, double **state, double **lam_in, double **lam_out, int rchunk,
+                 const double *bJ, double obj_r, double obj_rad)
 {
     HostBox hb{nr, nt, np, pitch, (long long)(nt + 1) * pitch};
     const BoxG b{nr, nt, np, hb.sj, hb.si};
@@ -139,7 +143,9 @@ int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const doub
             for (int k = 1; k <= np - 2; ++k)
                 if (next_to_boundary(b, i, j, k)) LB[b.off(i, j, k)] = lb_sum(a, i, j, k);
     a.LB = LB.data();
-    a.bJ = nullptr;
+    a.bJ = bJ;   // objective weights on the r-faces (box layout) or null
+    a.obj_r = obj_r;
+    a.obj_rad = obj_rad;
     const int rc = rchunk > 0 ? rchunk : nr + 1;
     a.rchunk = rc;
     const int nx = (np + 1 + AOK - 1) / AOK, ny = (nt + 1 + AOJ - 1) / AOJ, nz = (nr + 1 + rc - 1) / rc;

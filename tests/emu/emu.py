@@ -107,15 +107,17 @@ def _box1(setup, name, a):
     return b
 
 
-def adj_step(setup, state, lam, params, rchunk=0):
+def adj_step(setup, state, lam, params, rchunk=0, objective=None):
     """One application of the r-marching adjoint kernel on the host: state
-    (forward fields by name), lam (adjoint fields by name) -> new lam."""
+    (forward fields by name), lam (adjoint fields by name) -> new lam; with
+    objective = (r, squared radius, weights of VR's shape) the step also adds
+    the weights on the r-faces inside the inclusion."""
     L = lib()
     DP = C.POINTER(C.c_double)
     if not hasattr(L, "_adj_sig"):
         L.emu_adj_step.restype = C.c_int
         L.emu_adj_step.argtypes = [C.c_int] * 4 + [DP] * 3 + [C.POINTER(ss_params), C.POINTER(DP), C.POINTER(DP),
-                                                              C.POINTER(DP), C.c_int]
+                                                              C.POINTER(DP), C.c_int, DP, C.c_double, C.c_double]
         L._adj_sig = True
     sb = [_box1(setup, n, state[n]) for n in ADJ_STATE]
     li = [_box1(setup, n, lam[n]) for n in ADJ]
@@ -125,8 +127,13 @@ def adj_step(setup, state, lam, params, rchunk=0):
     po = (DP * len(lo))(*[x.ctypes.data_as(DP) for x in lo])
     sp = ss_params.from_params(params)
     R, th, ph = (np.ascontiguousarray(x, dtype=np.float64) for x in (setup.R, setup.th, setup.ph))
+    bj, r_obj, rad = None, 0.0, 0.0
+    if objective is not None:
+        r_obj, rad, w = objective
+        bj = _box1(setup, "VR", w)
     L.emu_adj_step(setup.nr, setup.nth, setup.nph, pitch_of(setup.nph), R.ctypes.data_as(DP), th.ctypes.data_as(DP),
-                   ph.ctypes.data_as(DP), C.byref(sp), ps, pi, po, int(rchunk))
+                   ph.ctypes.data_as(DP), C.byref(sp), ps, pi, po, int(rchunk),
+                   bj.ctypes.data_as(DP) if bj is not None else None, float(r_obj), float(rad))
     out = {}
     for n, x in zip(ADJ, lo):
         o = 1 if n in SHEAR else 0

@@ -45,3 +45,29 @@ def test_marching_adjoint_matches_reference(case, shape):
         scale = max(np.abs(ref[n]).max(), 1e-300)
         err = np.abs(got[n] - ref[n]).max() / scale
         assert err <= 1e-12, (n, err)
+
+
+@pytest.mark.parametrize("shape", [(11, 13, 70, 0), (11, 13, 70, 3), (9, 30, 7, 2)],
+                         ids=["11x13x70_r0", "11x13x70_r3", "9x30x7_r2"])
+def test_marching_adjoint_objective_matches_reference(shape):
+    """The step with the objective: T^T lam + dJ/dVR, dJ the oracle's
+    objective weights (centre r = 1.5, squared radius 0.1^2), which are zero
+    outside the inclusion; the kernel adds them where its own inclusion test
+    holds."""
+    nr, nth, nph, rch = shape
+    st = setups.diapir_sector(nr, nth=nth, nph=nph, n_exp=3.0, bc_mode=BC_FREE_SLIP)
+    setups.random_state(st, seed=11)
+    o = Oracle(st)
+    o.step(2)
+    _, dJ = o.objective(1.5, 0.1 ** 2)
+    assert np.count_nonzero(dJ) > 0
+    rng = np.random.default_rng(4)
+    lam = {n: rng.standard_normal(o.f[n].shape) for n in ORC_ADJ}
+    lam["eta_PL"][:] = 0.0
+    ref = o.adj_step(lam)
+    ref["VR"] = ref["VR"] + dJ
+    got = emu.adj_step(st, o.f, lam, st.params, rchunk=rch, objective=(1.5, 0.1 ** 2, dJ))
+    for n in ORC_ADJ:
+        scale = max(np.abs(ref[n]).max(), 1e-300)
+        err = np.abs(got[n] - ref[n]).max() / scale
+        assert err <= 1e-12, (n, err)
