@@ -732,7 +732,9 @@ int afail(ss_ctx *c, int code, const std::string &msg)
     return code;
 }
 
-int async(ss_ctx *c, const char *where)
+// Waits for the context's stream and returns the status of the work queued
+// on it (the entry points are blocking, as the reference's @parallel calls).
+int sync_status(ss_ctx *c, const char *where)
 {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipGetLastError();
@@ -898,7 +900,7 @@ int ss_adjoint_prepare(ss_ctx *c, const ss_params *p)
     hipLaunchKernelGGL(k_adj_prep_centre, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
     for (int fam = 0; fam < 3; ++fam)
         hipLaunchKernelGGL(k_adj_prep_edge, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a, fam);
-    return async(c, "ss_adjoint_prepare");
+    return sync_status(c, "ss_adjoint_prepare");
 }
 
 int ss_adjoint_zero(ss_ctx *c)
@@ -910,7 +912,7 @@ int ss_adjoint_zero(ss_ctx *c)
     hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GRHO], c->box_elems);
     hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GINI], c->box_elems);
     c->adj_cur = 0;
-    return async(c, "ss_adjoint_zero");
+    return sync_status(c, "ss_adjoint_zero");
 }
 
 int ss_adjoint_objective(ss_ctx *c, double r, double radius, double *J)
@@ -933,14 +935,14 @@ int ss_adjoint_objective(ss_ctx *c, double r, double radius, double *J)
     if (hipMemcpyAsync(sums, c->adj_part + 2 * OBJ_NBLK, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
         hipSuccess)
         return afail(c, SS_ERR_STATE, "objective readback");
-    int rc = async(c, "ss_adjoint_objective");
+    int rc = sync_status(c, "ss_adjoint_objective");
     if (rc != SS_OK) return rc;
     if (!(sums[0] > 0.0)) return afail(c, SS_ERR_ARG, "ss_adjoint_objective: no r-face inside the inclusion");
     const Box &b = c->box;
     hipLaunchKernelGGL(k_adj_scale, agrid(b.nr + 1, b.nt, b.np), AB, 0, c->stream, b, c->adj[AD_BJ], b.nr + 1, b.nt,
                        b.np, 1.0 / sums[0]);
     *J = sums[1] / sums[0];
-    return async(c, "ss_adjoint_objective");
+    return sync_status(c, "ss_adjoint_objective");
 }
 
 int ss_adjoint_upload(ss_ctx *c, int field, const double *host, int layout)
@@ -957,7 +959,7 @@ int ss_adjoint_upload(ss_ctx *c, int field, const double *host, int layout)
     if (hipMemcpyAsync(c->staging, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return afail(c, SS_ERR_STATE, "ss_adjoint_upload copy");
     launch_to_box(c, dst, c->staging, d[0], d[1], d[2], o1, layout);
-    return async(c, "ss_adjoint_upload");
+    return sync_status(c, "ss_adjoint_upload");
 }
 
 int ss_adjoint_download(ss_ctx *c, int field, double *host, int layout)
@@ -974,7 +976,7 @@ int ss_adjoint_download(ss_ctx *c, int field, double *host, int layout)
     launch_from_box(c, c->staging, src, d[0], d[1], d[2], o1, layout);
     if (hipMemcpyAsync(host, c->staging, n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         return afail(c, SS_ERR_STATE, "ss_adjoint_download copy");
-    return async(c, "ss_adjoint_download");
+    return sync_status(c, "ss_adjoint_download");
 }
 
 int ss_adjoint_step(ss_ctx *c, const ss_params *p, int gradients)
@@ -986,7 +988,7 @@ int ss_adjoint_step(ss_ctx *c, const ss_params *p, int gradients)
     a.grad = gradients ? 1 : 0;
     launch_adj_step(c, a, false);
     c->adj_cur = 1 - c->adj_cur;
-    return async(c, "ss_adjoint_step");
+    return sync_status(c, "ss_adjoint_step");
 }
 
 int ss_adjoint_solve(ss_ctx *c, const ss_params *p, double r, double radius, int maxiter, int ncheck, double tol,
@@ -1024,7 +1026,7 @@ int ss_adjoint_solve(ss_ctx *c, const ss_params *p, double r, double radius, int
             if (hipMemcpyAsync(c->red_host, c->red, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                c->stream) != hipSuccess)
                 return afail(c, SS_ERR_STATE, "adjoint check readback");
-            if ((rc = async(c, "adjoint iteration")) != SS_OK) return rc;
+            if ((rc = sync_status(c, "adjoint iteration")) != SS_OK) return rc;
             double err = 0.0;
             for (int q = 0; q < 3; ++q) {
                 double dv, vv;
@@ -1058,7 +1060,7 @@ int ss_adjoint_solve(ss_ctx *c, const ss_params *p, double r, double radius, int
     hipLaunchKernelGGL(k_adj_bc_R, G2, B2, 0, c->stream, a);
     hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
     hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
-    if ((rc = async(c, "adjoint gradients")) != SS_OK) return rc;
+    if ((rc = sync_status(c, "adjoint gradients")) != SS_OK) return rc;
     out->seconds = std::chrono::duration<double>(aclk::now() - t0).count();
     return out->converged ? SS_OK : SS_NOT_CONVERGED;
 }
