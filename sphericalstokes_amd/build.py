@@ -4,6 +4,7 @@
   * sphericalstokes_amd/libsspt.so   -- the product: HIP kernels + C ABI (hipcc, gfx950)
   * oracle/libsspt_oracle.so         -- CPU restatement used as parity checker (gcc)
   * tests/emu/libsspt_emu.so         -- host emulation of the fused kernels (g++)
+  * sphericalstokes_amd/sspt_run     -- command-line driver over the C ABI (g++)
 
 FP contraction is off everywhere (``-ffp-contract=off``) so that the device
 arithmetic rounds like the reference expressions; packed FP32 instructions
@@ -78,14 +79,29 @@ def build_emu() -> Path:
     return EMU_LIB
 
 
+CLI_SRC = PKG / "cli" / "sspt_run.cpp"
+CLI = PKG / "sspt_run"
+
+
+def build_cli() -> Path:
+    """The command-line driver over the C ABI (host code only, g++), linked
+    against libsspt.so in this directory."""
+    tmp = CLI.with_suffix(".tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", ROOT / "include", CLI_SRC, "-L", PKG, "-lsspt",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp])
+    os.replace(tmp, CLI)
+    return CLI
+
+
 def build_all() -> None:
     build_oracle()
     build_emu()
     build_lib()
+    build_cli()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "lib": build_lib, "oracle": build_oracle, "emu": build_emu,
+        {"all": build_all, "lib": build_lib, "oracle": build_oracle, "emu": build_emu, "cli": build_cli,
          "usage": lambda: build_lib(verbose_asm=True)}[w]()

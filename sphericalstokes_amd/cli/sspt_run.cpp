@@ -1,0 +1,191 @@
+// sspt_run -- command-line driver of the PT solve through the C ABI of
+// libsspt.so (include/sspt.h): the main program of
+// scripts_diapir/SphericalStokes_diapir.jl (SPH_3D_Inclusion, :347-453) as a
+// compiled caller.  Sets up the rising-sphere diapir, runs the PT loop with
+// the reference exit test (ss_pt_solve) and prints the script's performance
+// summary (:445-453): total iterations, wall time of iterations 11..ittot and
+// T_eff = A_eff / t_it with A_eff = 26 n^3 8 B.
+//
+// Setups (the formulas of sphericalstokes_amd/setups.py: diapir, diapir_sector):
+//   --setup sector     r in [1,2], th in [pi/3, 2pi/3], ph in [-pi/6, pi/6],
+//                      sphere of squared radius 0.01 at (1.5, pi/2, 0), drho = -1,
+//                      g = 1, eta_in = 1/vr, free slip (default)
+//   --setup reference  the script's own: r = 1000, lr = 6, lth = lph = lr/r,
+//                      radius 1, drho = -10, n = 5, simple shear sh = 1
+// Coordinates are range(a, b, length = n) as numpy.linspace evaluates it:
+// a + i * ((b - a) / (n - 1)), the last one b.
+#include "sspt.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+std::vector<double> linspace(double a, double b, int n)
+{
+    std::vector<double> v(n);
+    const double step = (b - a) / (double)(n - 1);
+    for (int i = 0; i < n; ++i) v[i] = (double)i * step + a;
+    if (n > 1) v[n - 1] = b;
+    return v;
+}
+
+struct Opts {
+    int nr = 63, nth = 0, nph = 0, device = 0;
+    std::string setup = "sector";
+    double n_exp = -1.0, vr = -1.0, sh = -1.0, eps = 1e-8;
+    int maxiter = 100000, nout = 100, bc = -1, timed = 0, warmup = 10;
+    bool json = false;
+    std::string perf_file;
+};
+
+void usage()
+{
+    std::printf(
+        "usage: sspt_run [--n N | --nr N --nth N --nph N] [--setup sector|reference] [--nexp X] [--vr X]\n"
+        "                [--bc free_slip|shear|pure_shear] [--sh X] [--maxiter M] [--nout K] [--eps E]\n"
+        "                [--timed K [--warmup W]] [--device D] [--json] [--perf-file PATH]\n"
+        "  --timed K: W untimed, then K fused iterations with HIP-event timing instead of the solve\n"
+        "  --perf-file: append the reference's out_perf record (nr nth nph dmp ittot wtime A_eff t_it T_eff)\n");
+}
+
+bool parse(int argc, char **argv, Opts &o)
+{
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto val = [&](void) -> const char * { return i + 1 < argc ? argv[++i] : nullptr; };
+        const char *v = nullptr;
+        if (a == "--help" || a == "-h") { usage(); std::exit(0); }
+        else if (a == "--json") o.json = true;
+        else if (!(v = val())) { std::fprintf(stderr, "sspt_run: %s needs a value\n", a.c_str()); return false; }
+        else if (a == "--n") o.nr = o.nth = o.nph = std::atoi(v);
+        else if (a == "--nr") o.nr = std::atoi(v);
+        else if (a == "--nth") o.nth = std::atoi(v);
+        else if (a == "--nph") o.nph = std::atoi(v);
+        else if (a == "--setup") o.setup = v;
+        else if (a == "--nexp") o.n_exp = std::atof(v);
+        else if (a == "--vr") o.vr = std::atof(v);
+        else if (a == "--sh") o.sh = std::atof(v);
+        else if (a == "--eps") o.eps = std::atof(v);
+        else if (a == "--maxiter") o.maxiter = std::atoi(v);
+        else if (a == "--nout") o.nout = std::atoi(v);
+        else if (a == "--timed") o.timed = std::atoi(v);
+        else if (a == "--warmup") o.warmup = std::atoi(v);
+        else if (a == "--device") o.device = std::atoi(v);
+        else if (a == "--perf-file") o.perf_file = v;
+        else if (a == "--bc") {
+            const std::string b = v;
+            o.bc = b == "shear" ? SS_BC_SHEAR : b == "free_slip" ? SS_BC_FREE_SLIP : b == "pure_shear" ? SS_BC_PURE_SHEAR : -2;
+            if (o.bc == -2) { std::fprintf(stderr, "sspt_run: unknown --bc %s\n", v); return false; }
+        } else { std::fprintf(stderr, "sspt_run: unknown option %s\n", a.c_str()); return false; }
+    }
+    if (o.nth <= 0) o.nth = o.nr;
+    if (o.nph <= 0) o.nph = o.nr;
+    if (o.setup != "sector" && o.setup != "reference") { std::fprintf(stderr, "sspt_run: unknown --setup\n"); return false; }
+    return true;
+}
+
+int fail(ss_ctx *c, int rc, const char *what)
+{
+    std::fprintf(stderr, "sspt_run: %s failed (status %d): %s\n", what, rc, c ? ss_last_error(c) : "");
+    if (c) ss_destroy(c);
+    return 1;
+}
+
+} // namespace
+
+int main(int argc, char **argv)
+{
+    Opts o;
+    if (!parse(argc, argv, o)) { usage(); return 2; }
+    const bool sector = o.setup == "sector";
+    // the setup's constants (setups.diapir_sector / setups.diapir defaults)
+    const double r = sector ? 1.5 : 1000.0, lr = sector ? 1.0 : 6.0;
+    const double lth = sector ? M_PI / 3.0 : lr / r, lph = sector ? M_PI / 3.0 : lr / r;
+    const double radius = sector ? 0.1 * 0.1 : 1.0, eta0 = 1.0, rho0 = 0.0, drho = sector ? -1.0 : -10.0;
+    const double vr = o.vr > 0 ? o.vr : (sector ? 1e3 : 1e2);
+    const double n_exp = o.n_exp > 0 ? o.n_exp : (sector ? 1.0 : 5.0);
+    const double sh = o.sh >= 0 ? o.sh : (sector ? 0.0 : 1.0);
+    const int bc = o.bc >= 0 ? o.bc : (sector ? SS_BC_FREE_SLIP : SS_BC_SHEAR);
+    const int nr = o.nr, nth = o.nth, nph = o.nph;
+    if (nr < 3 || nth < 3 || nph < 3) { std::fprintf(stderr, "sspt_run: sizes below 3\n"); return 2; }
+
+    ss_ctx *c = nullptr;
+    int rc = ss_create(nr, nth, nph, o.device, &c);
+    if (rc != SS_OK) return fail(nullptr, rc, "ss_create");
+    const std::vector<double> R = linspace(r - lr / 2.0, r + lr / 2.0, nr);
+    const std::vector<double> th = linspace(M_PI / 2.0 - lth / 2.0, M_PI / 2.0 + lth / 2.0, nth);
+    const std::vector<double> ph = linspace(-lph / 2.0, lph / 2.0, nph);
+    if ((rc = ss_set_grid(c, R.data(), th.data(), ph.data())) != SS_OK) return fail(c, rc, "ss_set_grid");
+    // rho = rho0, eta = eta0, the inclusion, eta_ini = eta_PL = eta (:380-404)
+    if ((rc = ss_fill(c, SS_RHO, rho0)) != SS_OK) return fail(c, rc, "ss_fill(rho)");
+    if ((rc = ss_fill(c, SS_ETA, eta0)) != SS_OK) return fail(c, rc, "ss_fill(eta)");
+    if ((rc = ss_initialize_inclusion(c, SS_RHO, r, radius, rho0 + drho)) != SS_OK) return fail(c, rc, "inclusion(rho)");
+    if ((rc = ss_initialize_inclusion(c, SS_ETA, r, radius, eta0 / vr)) != SS_OK) return fail(c, rc, "inclusion(eta)");
+    if ((rc = ss_copy_field(c, SS_ETA_INI, SS_ETA)) != SS_OK) return fail(c, rc, "eta_ini .= eta");
+    if ((rc = ss_copy_field(c, SS_ETA_PL, SS_ETA)) != SS_OK) return fail(c, rc, "eta_PL .= eta");
+    // the far-field shear flow as the initial guess (initialize_velocity!, :403)
+    if (bc == SS_BC_SHEAR && (rc = ss_initialize_velocity(c, sh, r)) != SS_OK) return fail(c, rc, "initialize_velocity");
+
+    ss_params p;
+    std::memset(&p, 0, sizeof p);
+    const double CFL = 1.0 / (2.0 + 4.5 * std::log10(vr));
+    p._dr = (nr - 1) / lr;
+    p._dth = (nth - 1) / lth;
+    p._dph = (nph - 1) / lph;
+    p.dmp = 4.5;
+    p.vpdt = (lr / (nr - 1)) * CFL;
+    p.lr = lr;
+    p.tau_C = 1.0;
+    p.n_exp = n_exp;
+    p.g = 1.0;
+    p.r = r;
+    p.sh = sh;
+    p.relax = 1.0;
+    p.eta_min = 0.0;
+    p.eta_max = INFINITY;
+    p.bc_mode = bc;
+    const double A_eff = 26.0 * (double)nr * nth * nph * 8.0 / 1e9;   // GB (:447)
+
+    std::printf("Starting calculation (nr=%d, nth=%d , nph=%d), setup %s, n_exp %g, vr %g\n", nr, nth, nph,
+                o.setup.c_str(), n_exp, vr);
+    if (o.timed > 0) {
+        double main_ms = 0.0, iter_ms = 0.0;
+        if (o.warmup > 0 && (rc = ss_pt_iterate(c, &p, o.warmup)) != SS_OK) return fail(c, rc, "ss_pt_iterate (warm-up)");
+        if ((rc = ss_pt_iterate_timed(c, &p, o.timed, &main_ms, &iter_ms)) != SS_OK) return fail(c, rc, "ss_pt_iterate_timed");
+        const double T_eff = A_eff / (iter_ms * 1e-3);
+        std::printf("%d fused iterations: %.4f ms per iteration (main kernel %.4f ms) @ T_eff = %.2f GB/s\n", o.timed,
+                    iter_ms, main_ms, T_eff);
+        if (o.json)
+            std::printf("{\"n\": [%d, %d, %d], \"iterations\": %d, \"iter_ms\": %.6f, \"main_ms\": %.6f, \"T_eff_GBps\": %.3f}\n",
+                        nr, nth, nph, o.timed, iter_ms, main_ms, T_eff);
+        ss_destroy(c);
+        return 0;
+    }
+    ss_result res;
+    std::memset(&res, 0, sizeof res);
+    rc = ss_pt_solve(c, &p, o.maxiter, o.nout, o.eps, &res);
+    if (rc != SS_OK && rc != SS_NOT_CONVERGED) return fail(c, rc, "ss_pt_solve");
+    const double wtime = res.seconds, t_it = res.ittot > 10 ? wtime / (res.ittot - 10) : NAN;
+    const double T_eff = A_eff / t_it;
+    std::printf("iter %d, err=%1.3e %s\n", res.iter_end ? res.iter_end : res.ittot, res.err,
+                rc == SS_OK ? "" : "(not converged)");
+    std::printf("Total iters = %d (%d steps), time = %1.3e sec (@ T_eff = %1.2f GB/s) \n", res.ittot, 1, wtime, T_eff);
+    if (!o.perf_file.empty()) {
+        if (FILE *f = std::fopen(o.perf_file.c_str(), "a")) {
+            std::fprintf(f, "%d %d %d %g %d %.17g %.17g %.17g %.17g\n", nr, nth, nph, p.dmp, res.ittot, wtime, A_eff, t_it,
+                         T_eff);
+            std::fclose(f);
+        }
+    }
+    if (o.json)
+        std::printf("{\"n\": [%d, %d, %d], \"ittot\": %d, \"iter_end\": %d, \"err\": %.17g, \"converged\": %s, "
+                    "\"seconds\": %.6f, \"T_eff_GBps\": %.3f}\n",
+                    nr, nth, nph, res.ittot, res.iter_end, res.err, rc == SS_OK ? "true" : "false", wtime, T_eff);
+    ss_destroy(c);
+    return rc == SS_OK ? 0 : 3;
+}
