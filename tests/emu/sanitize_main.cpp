@@ -49,7 +49,8 @@ extern "C" int emu_fused_iterations(int nr, int nt, int np, int pitch, const dou
                                     const double *ph1, const ss_params *p, double **cur, double **alt, int niter,
                                     int check_last, int rchunk, unsigned long long red[6], long long pair_bytes);
 extern "C" int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const double *th1, const double *ph1,
-                            const ss_params *p, double **state, double **lam_in, double **lam_out, int rchunk);
+                            const ss_params *p, double **state, double **lam_in, double **lam_out, int rchunk,
+                            const double *bJ, double obj_r, double obj_rad);
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -223,6 +224,11 @@ int run_case(const Case &c)
     std::vector<std::vector<double>> lam_ref = lam;
     for (int q = 0; q < NA; ++q) oa.a[q] = lam_ref[q].data();
     orc_adj_step(&os, &pp, &oa, nullptr, nullptr);
+    // with the objective (mean VR in the sphere of squared radius 0.01 at r = 1.5):
+    // its weights are added on the r-faces inside
+    std::vector<double> dJ(O[SS_VR].size(), 0.0);
+    (void)orc_adj_objective(&os, 1.5, 0.01, dJ.data());
+    for (size_t q = 0; q < dJ.size(); ++q) lam_ref[ORC_A_VR][q] += dJ[q];
     const int st_field[12] = {SS_TAU_RR, SS_TAU_THTH, SS_TAU_PHPH, SS_TAU_RTH, SS_TAU_RPH, SS_TAU_THPH,
                               SS_ETA, SS_ETA_INI, SS_ETA_PL, SS_TII, SS_DT_R, SS_ETASM};
     double *sb[12], *li[NA], *lo[NA];
@@ -244,7 +250,8 @@ int run_case(const Case &c)
         lo[q] = alloc_region(box);
         owned2.push_back(lo[q]);
     }
-    emu_adj_step(nr, nt, np, pitch, R.data(), th.data(), ph.data(), &p, sb, li, lo, c.rchunk);
+    double *bJ = to_box(dJ, SS_VR);
+    emu_adj_step(nr, nt, np, pitch, R.data(), th.data(), ph.data(), &p, sb, li, lo, c.rchunk, bJ, 1.5, 0.01);
     for (int q = 0; q < NA; ++q) {
         const Shape s = shape_of(adj_field[q], nr, nt, np);
         double num = 0.0, mx = 0.0;
