@@ -15,6 +15,11 @@
  * of the other directions.  Below they are written out at 1-based
  * (ix, iy, iz); the averages sum their terms in the order
  * (ix,iy) (ix+1,iy) (ix,iy+1) (ix+1,iy+1) of the pair of directions.
+ * ParallelStencil (a dependency, not part of the reference) defines those
+ * macros; the vectors are evaluated with the same restatement of them
+ * (tests/golden/jl_subset.py), so the term order inside a macro is pinned
+ * only by that restatement ("parity unpinned" for it); the statements, their
+ * association outside the macros, the guards and the loop are the script's.
  */
 #include "sspt_cyl_oracle.h"
 #include <math.h>
