@@ -58,13 +58,14 @@ __device__ __forceinline__ void tile_of(int nx, int ny, int nz, int &bx, int &by
 #endif
 
 template <typename R, int FL>
-__global__ void __launch_bounds__(TK * TJB<R>) SSPT_MAIN_ATTR k_fused_main(Args<R> a, int nx, int ny, int nz)
+__global__ void __launch_bounds__(TKX<R> * TJB<R>) SSPT_MAIN_ATTR k_fused_main(Args<R> a, int nx, int ny, int nz)
 {
     __shared__ Lds<R> L;
     int bx, by, bz;
     tile_of(nx, ny, nz, bx, by, bz);
     if (bz >= nz) return; // padding slots of the remap (uniform per block)
-    const int kb = bx * OK_, jb = by * OJ<R>;
+    static_assert(TKX<R> % 64 == 0 && TKX<R> * TJB<R> <= 1024, "rows of whole wavefronts, at most 1024 threads");
+    const int kb = bx * OKX<R>, jb = by * OJ<R>;
     const int s0 = bz * a.rchunk, s1 = min(s0 + a.rchunk, a.b.nr);
     const int start = max(s0 - 2, -1);
     Thread<R> T;
@@ -82,7 +83,7 @@ __global__ void __launch_bounds__(TK * TJB<R>) SSPT_MAIN_ATTR k_fused_main(Args<
         // VR/VT/VP/ETA, so no barrier is needed between them.
         phase4<R, FL>(T, L, a, s, wr);
     }
-    if (FL & F_CHECK) flush_max(a, T.mdv, T.mv, threadIdx.x);
+    if (FL & F_CHECK) flush_max(a, T.mdv, T.mv, threadIdx.x & 63);   // lane within the wavefront
 }
 
 // Boundary pass.  blockIdx.z = family: 0 -> r faces over (j,k),
@@ -351,7 +352,7 @@ template <typename R>
 void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_main1)
 {
     a.pp_in = a.pp_out = a.ppe_in = a.ppe_out = 0;
-    const int nx = (a.b.np + OK_ - 1) / OK_;
+    const int nx = (a.b.np + OKX<R> - 1) / OKX<R>;
     const int ny = (a.b.nt + OJ<R> - 1) / OJ<R>;
     a.rchunk = pick_rchunk(c, a.b.nr, nx * ny, sizeof(R) == 4);
     const int nz = (a.b.nr + a.rchunk - 1) / a.rchunk;
@@ -363,7 +364,7 @@ void launch_fused_args(ss_ctx *c, Args<R> a, hipEvent_t ev_main0, hipEvent_t ev_
     const int T = nx * ny * nz;
     const int grid = ((T + 7) / 8) * 8;
     if (ev_main0) (void)hipEventRecord(ev_main0, c->stream);
-    const dim3 G(grid), B(TK, TJB<R>);
+    const dim3 G(grid), B(TKX<R>, TJB<R>);
     Args<R> am = a; // main-kernel view; the boundary pass keeps the plain pointers
     if (c->pair_state) pair_args(am, (long long)(c->box_elems * sizeof(R)));
     if constexpr (sizeof(R) == 8) {

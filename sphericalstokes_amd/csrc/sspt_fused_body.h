@@ -36,7 +36,17 @@
 namespace sspt {
 namespace fused {
 
-constexpr int TK = 64;        // lanes along phi (one wavefront)
+constexpr int TK = 64;        // lanes along phi of the adjoint's blocks (one wavefront)
+// lanes along phi per row of the PT-iteration blocks, a multiple of 64 (a row
+// is one or more whole wavefronts; every exchange between lanes goes through
+// LDS across a barrier, so a row may span several)
+#ifndef SSPT_TK64
+#define SSPT_TK64 64
+#endif
+#ifndef SSPT_TK32
+#define SSPT_TK32 64
+#endif
+template <typename R> constexpr int TKX = sizeof(R) == 8 ? SSPT_TK64 : SSPT_TK32;
 // rows along th: 12 for FP64 (3 wavefronts per SIMD at <= 168 VGPRs), 16 for
 // FP32 (4 per SIMD)
 #ifndef SSPT_TJB64
@@ -46,7 +56,7 @@ constexpr int TK = 64;        // lanes along phi (one wavefront)
 #define SSPT_TJB32 16
 #endif
 template <typename R> constexpr int TJB = sizeof(R) == 8 ? SSPT_TJB64 : SSPT_TJB32;
-constexpr int OK_ = TK - 2;   // owned points per row
+template <typename R> constexpr int OKX = TKX<R> - 2;   // owned points per row
 template <typename R> constexpr int OJ = TJB<R> - 2; // owned rows
 constexpr double PI_ = 3.14159265358979323846;
 
@@ -230,11 +240,11 @@ template <typename R, int FL> SSPT_DH inline void st_out(R *out, const R *in, ui
 }
 
 template <typename R> struct Lds {
-    R VR[TJB<R>][TK], VT[TJB<R>][TK], VP[TJB<R>][TK];
-    R ETA[TJB<R> + 2][TK + 2];
-    R SM[TJB<R>][TK];
-    R STT[TJB<R>][TK], SPP[TJB<R>][TK], DT[TJB<R>][TK];
-    R RT[TJB<R>][TK], RP[TJB<R>][TK], TP[TJB<R>][TK];
+    R VR[TJB<R>][TKX<R>], VT[TJB<R>][TKX<R>], VP[TJB<R>][TKX<R>];
+    R ETA[TJB<R> + 2][TKX<R> + 2];
+    R SM[TJB<R>][TKX<R>];
+    R STT[TJB<R>][TKX<R>], SPP[TJB<R>][TKX<R>], DT[TJB<R>][TKX<R>];
+    R RT[TJB<R>][TKX<R>], RP[TJB<R>][TKX<R>], TP[TJB<R>][TKX<R>];
 };
 
 // Normal velocity that the pure-shear BC prescribes on a th boundary at th_b:
@@ -355,7 +365,7 @@ template <typename R> SSPT_DH inline void thread_init(Thread<R> &T, const Args<R
     T.j = SSPT_UNIFORM(jb - 1 + ty);
     T.kc = clampi(T.k, 0, a.b.np);
     T.jc = clampi(T.j, 0, a.b.nt);
-    T.own = tx >= 1 && tx <= TK - 2 && ty >= 1 && ty <= TJB<R> - 2 && T.j < a.b.nt && T.k < a.b.np;
+    T.own = tx >= 1 && tx <= TKX<R> - 2 && ty >= 1 && ty <= TJB<R> - 2 && T.j < a.b.nt && T.k < a.b.np;
     for (int c = 0; c < 3; ++c) T.mdv[c] = T.mv[c] = 0ull;
     T.vr0 = T.vr1 = T.vt0 = T.vt1 = T.vp0 = T.vp1 = R(0);
     T.e0 = T.e1 = T.e2 = R(1);
@@ -376,7 +386,7 @@ template <typename R> SSPT_DH inline void load_ring(Thread<R> &T, const Args<R> 
     if (T.ty == 0) T.ringJ = fat_in(a.eta, a.ppe_in, b.o32(ie, clampi(T.j - 1, 0, b.nt - 1), kcc));
     if (T.ty == TJB<R> - 1) T.ringJ = fat_in(a.eta, a.ppe_in, b.o32(ie, clampi(T.j + 1, 0, b.nt - 1), kcc));
     if (T.tx == 0) T.ringK = fat_in(a.eta, a.ppe_in, b.o32(ie, jcc, clampi(T.k - 1, 0, b.np - 1)));
-    if (T.tx == TK - 1) T.ringK = fat_in(a.eta, a.ppe_in, b.o32(ie, jcc, clampi(T.k + 1, 0, b.np - 1)));
+    if (T.tx == TKX<R> - 1) T.ringK = fat_in(a.eta, a.ppe_in, b.o32(ie, jcc, clampi(T.k + 1, 0, b.np - 1)));
 }
 
 template <typename R> SSPT_DH inline uint32_t own_centre(const Thread<R> &T, const BoxG &b, int s)
@@ -464,7 +474,7 @@ SSPT_DH inline void phase1(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s)
     if (T.ty == 0) L.ETA[0][T.tx + 1] = T.ringJ;
     if (T.ty == TJB<R> - 1) L.ETA[TJB<R> + 1][T.tx + 1] = T.ringJ;
     if (T.tx == 0) L.ETA[T.ty + 1][0] = T.ringK;
-    if (T.tx == TK - 1) L.ETA[T.ty + 1][TK + 1] = T.ringK;
+    if (T.tx == TKX<R> - 1) L.ETA[T.ty + 1][TKX<R> + 1] = T.ringK;
     // prefetch for step s+1
     const int jcc = clampi(T.jc, 0, b.nt - 1), kcc = clampi(T.kc, 0, b.np - 1);
     T.nvr = fat_in(a.VR, a.pp_in, b.o32(clampi(s + 2, 0, nr), T.jc, T.kc));
@@ -551,7 +561,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
             T.sm1 = star(L, T.ty, T.tx, T.e0, T.e2);
         } else {
             const int dty = cj - T.j, dtx = ck - T.k;
-            const int ty = clampi(T.ty + dty, 0, TJB<R> - 1), tx = clampi(T.tx + dtx, 0, TK - 1);
+            const int ty = clampi(T.ty + dty, 0, TJB<R> - 1), tx = clampi(T.tx + dtx, 0, TKX<R> - 1);
             const R em = fat_in(a.eta, a.ppe_in, b.o32(p - 1, cj, ck)), ep = fat_in(a.eta, a.ppe_in, b.o32(p + 1, cj, ck));
             T.sm1 = star(L, ty, tx, em, ep);
         }
@@ -565,7 +575,7 @@ SSPT_DH inline void phase2(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
 
     const bool centre = T.j >= 0 && T.j < nt && T.k >= 0 && T.k < np;
     // C(s): compute_P! centre part
-    if (s >= 0 && s <= nr - 1 && centre && T.tx < TK - 1 && T.ty < TJB<R> - 1) {
+    if (s >= 0 && s <= nr - 1 && centre && T.tx < TKX<R> - 1 && T.ty < TJB<R> - 1) {
         const R Rv = tload(t.tr, s * NTR + TR_Rc), iR = tload(t.tr, s * NTR + TR_invRc);
         const R dVRr = (T.vr1 - T.vr0) * a._dr;
         const R dVTt = (iR * (L.VT[T.ty + 1][T.tx] - T.vt0)) * a._dth;
@@ -795,7 +805,7 @@ SSPT_DH inline void phase4(Thread<R> &T, Lds<R> &L, const Args<R> &a, int s, boo
     }
     // carries for step s+1
     if (T.ty + 1 < TJB<R>) T.rt0n = L.RT[T.ty + 1][T.tx];
-    if (T.tx + 1 < TK) T.rp0n = L.RP[T.ty][T.tx + 1];
+    if (T.tx + 1 < TKX<R>) T.rp0n = L.RP[T.ty][T.tx + 1];
     T.rt0 = T.nrt;
     T.rp0 = T.nrp;
     T.cRR = T.nSRR;

@@ -35,21 +35,21 @@ struct HostBox {
 template <typename R, int FL>
 void run_main_t(const Args<R> &a, int rchunk, unsigned long long red[6])
 {
-    const int nx = (a.b.np + OK_ - 1) / OK_;
+    const int nx = (a.b.np + OKX<R> - 1) / OKX<R>;
     const int ny = (a.b.nt + OJ<R> - 1) / OJ<R>;
     const int nz = (a.b.nr + rchunk - 1) / rchunk;
     std::unique_ptr<Lds<R>> L(new Lds<R>());
-    std::vector<Thread<R>> th(TK * TJB<R>);
+    std::vector<Thread<R>> th(TKX<R> * TJB<R>);
     for (int bz = 0; bz < nz; ++bz)
         for (int by = 0; by < ny; ++by)
             for (int bx = 0; bx < nx; ++bx) {
                 memset(L.get(), 0xff, sizeof(Lds<R>)); // NaN garbage: unset LDS reads must not reach outputs
-                const int kb = bx * OK_, jb = by * OJ<R>;
+                const int kb = bx * OKX<R>, jb = by * OJ<R>;
                 const int s0 = bz * rchunk, s1 = std::min(s0 + rchunk, a.b.nr);
                 const int start = std::max(s0 - 2, -1);
                 for (int ty = 0; ty < TJB<R>; ++ty)
-                    for (int tx = 0; tx < TK; ++tx) {
-                        Thread<R> &T = th[ty * TK + tx];
+                    for (int tx = 0; tx < TKX<R>; ++tx) {
+                        Thread<R> &T = th[ty * TKX<R> + tx];
                         thread_init(T, a, tx, ty, kb, jb);
                         thread_prologue<R, FL>(T, a, start);
                     }
