@@ -8,11 +8,11 @@
 // faces of V (the BC_R!/BC_th!/BC_ph! sequence) and the boundary-cell
 // viscosities (power law of the copied tII).
 //
-// Block = 64 lanes along phi x TJB rows along th (12 in FP64, 16 in FP32; one
-// wavefront per row).  Lane tx covers k = kb-1+tx, row ty covers j = jb-1+ty;
-// the 62 x (TJB-2) inner points are owned (written), lane 0 / row 0 are a low
-// halo and lane 63 / row TJB-1 a high halo that are
-// recomputed from the old state.  The block marches along r: step s handles
+// Block = TKX lanes along phi x TJB rows along th (64 x 12 in FP64, 64 x 16 in
+// FP32; a row is one wavefront at TKX = 64, several at a wider build option).
+// Lane tx covers k = kb-1+tx, row ty covers j = jb-1+ty; the (TKX-2) x (TJB-2)
+// inner points are owned (written), lane 0 / row 0 are a low halo and lane
+// TKX-1 / row TJB-1 a high halo that are recomputed from the old state.  The block marches along r: step s handles
 // centre plane s, r-face s (VR) and r-face s+1 (tau_Rth, tau_Rph).  Values of
 // the current plane that neighbours need go through LDS; values the same
 // column needs at the next plane stay in registers (the Carry struct).
