@@ -13,7 +13,10 @@
 //   --setup reference  the script's own: r = 1000, lr = 6, lth = lph = lr/r,
 //                      radius 1, drho = -10, n = 5, simple shear sh = 1
 // Coordinates are range(a, b, length = n) as numpy.linspace evaluates it:
-// a + i * ((b - a) / (n - 1)), the last one b.
+// a + i * ((b - a) / (n - 1)), the last one b.  With --bc shear the initial
+// flow is the far-field simple shear (initialize_velocity!); the other modes
+// start from rest (setups.py seeds the pure-shear flow instead, so iteration
+// counts differ there, the converged state does not).
 #include "sspt.h"
 
 #include <cmath>
@@ -50,6 +53,7 @@ void usage()
         "                [--bc free_slip|shear|pure_shear] [--sh X] [--maxiter M] [--nout K] [--eps E]\n"
         "                [--timed K [--warmup W]] [--device D] [--json] [--perf-file PATH]\n"
         "  --timed K: W untimed, then K fused iterations with HIP-event timing instead of the solve\n"
+        "  --bc pure_shear starts from rest; shear from the far-field simple shear\n"
         "  --perf-file: append the reference's out_perf record (nr nth nph dmp ittot wtime A_eff t_it T_eff)\n");
 }
 
