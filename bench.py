@@ -98,10 +98,15 @@ def adjoint_iteration_ms(s, p, steps, r, radius):
     return (time.perf_counter() - t0) / steps * 1e3
 
 
-def adjoint_solve_ratio(n, eps):
+def adjoint_solve_ratio(n, eps, mixed=True):
     """Forward solve (reference exit test, relative change of V < eps every
     500 iterations) and adjoint solve (relative change of the adjoint < eps)
-    of the power-law diapir_sector at n^3, free slip: wall times and ratio."""
+    of the power-law diapir_sector at n^3, free slip: wall times and ratio.
+    With mixed, also the adjoint solve with FP32 sweeps inside the FP64
+    defect correction (ss_adjoint_solve_mixed) from the same state.  The
+    adjoint times are the library's own (prepare, objective, iterations,
+    gradients; the downloads of the gradient arrays are left out)."""
+    import numpy as np
     from sphericalstokes_amd import setups
     from sphericalstokes_amd.fields import BC_FREE_SLIP, FIELDS
     from sphericalstokes_amd.solver import Solver
@@ -116,14 +121,24 @@ def adjoint_solve_ratio(n, eps):
     f = s.pt_solve(p, 4_000_000, 500, eps)
     tf = time.perf_counter() - t0
     print(f"  adjoint {n}^3: forward {f['ittot']} iterations in {tf:.2f} s", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
     a = s.adjoint_solve(p, 1.5, 0.1 ** 2, maxiter=4_000_000, ncheck=500, tol=eps)
-    ta = time.perf_counter() - t0
+    ta = a["seconds"]
+    print(f"  adjoint {n}^3: FP64 adjoint {a['iters']} iterations in {ta:.2f} s", file=sys.stderr, flush=True)
+    out = {"workload": f"diapir_sector {n}^3, n_exp=3, free slip; J = mean VR in the inclusion", "eps": eps,
+           "forward_s": round(tf, 3), "forward_iters": f["ittot"], "adjoint_s": round(ta, 3),
+           "adjoint_iters": a["iters"], "adjoint_converged": a["converged"],
+           "ratio_adjoint_over_forward": round(ta / tf, 4)}
+    if mixed:
+        m = s.adjoint_solve_mixed(p, 1.5, 0.1 ** 2, nout_inner=500, tol=eps)
+        print(f"  adjoint {n}^3: mixed adjoint {m['outer']} outer steps, {m['inner_total']} FP32 sweeps in "
+              f"{m['seconds']:.2f} s", file=sys.stderr, flush=True)
+        diff = max(float(np.abs(m[k] - a[k]).max() / np.abs(a[k]).max()) for k in ("grad_rho", "grad_log10_eta_ini"))
+        out.update({"mixed_adjoint_s": round(m["seconds"], 3), "mixed_outer": m["outer"],
+                    "mixed_fp32_sweeps": m["inner_total"], "mixed_converged": m["converged"],
+                    "mixed_grad_rel_diff_vs_fp64": float(f"{diff:.3e}"),
+                    "ratio_mixed_adjoint_over_forward": round(m["seconds"] / tf, 4)})
     s.close()
-    return {"workload": f"diapir_sector {n}^3, n_exp=3, free slip; J = mean VR in the inclusion", "eps": eps,
-            "forward_s": round(tf, 3), "forward_iters": f["ittot"], "adjoint_s": round(ta, 3),
-            "adjoint_iters": a["iters"], "adjoint_converged": a["converged"],
-            "ratio_adjoint_over_forward": round(ta / tf, 4)}
+    return out
 
 
 def time_to_tolerance(n, tol, tol_inner, setup="diapir"):

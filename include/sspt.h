@@ -238,6 +238,27 @@ int ss_adjoint_step(ss_ctx *ctx, const ss_params *p, int gradients);
  * was reached (the gradients are those of the last lam). */
 int ss_adjoint_solve(ss_ctx *ctx, const ss_params *p, double r, double radius, int maxiter, int ncheck, double tol,
                      ss_adjoint_result *out);
+/* The same solve with FP32 sweeps inside an FP64 defect correction (no
+ * reference counterpart; the counterpart of ss_pt_solve_mixed).  It iterates
+ * the BC-folded lam (mu = F lam, F the transpose of the BC updates; the same
+ * fixed point).  Each outer step applies one FP64 step Y = F(T^T mu + dJ/dx)
+ * and stops when max|Y - mu|/max|Y| over the velocity components < o->tol;
+ * otherwise it iterates the correction d = F T^T d + (Y - mu) with the
+ * r-marching kernel in FP32 (state, tables and d rounded to FP32) until
+ * max|dd|/max|d| < the inner tolerance (checked every o->nout_inner sweeps)
+ * or o->max_inner sweeps, then mu += F d in FP64.  The inner tolerance is
+ * o->tol_inner, loosened (to at most 1e-2) when the reduction still needed
+ * is smaller than the last outer step's.  Gradients as ss_adjoint_solve.
+ * Allocates 49 FP32 boxes on first use. */
+typedef struct ss_adjoint_mixed_result {
+    int outer, inner_total, converged;
+    double J;              /* objective at the state */
+    double err;            /* max|Y - mu|/max|Y| of the last outer step */
+    double seconds;
+    double inner_seconds;  /* part of seconds spent in the FP32 sweeps */
+} ss_adjoint_mixed_result;
+int ss_adjoint_solve_mixed(ss_ctx *ctx, const ss_params *p, double r, double radius, const ss_mixed_opts *o,
+                           ss_adjoint_mixed_result *out);
 
 /* ---- timing hooks for the benchmark (no reference counterpart) ---- */
 /* Average device time (ms) per iteration of the last ss_pt_iterate, from HIP

@@ -58,6 +58,11 @@ class ss_adjoint_result(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class ss_adjoint_mixed_result(C.Structure):
+    _fields_ = [("outer", C.c_int), ("inner_total", C.c_int), ("converged", C.c_int), ("J", C.c_double),
+                ("err", C.c_double), ("seconds", C.c_double), ("inner_seconds", C.c_double)]
+
+
 # adjoint field ids (enum ss_adj_field)
 ADJ_FIELDS = ["VR", "Vth", "Vph", "P", "tau_RR", "tau_thth", "tau_phph", "tau_Rth", "tau_Rph", "tau_thph", "eta",
               "eta_PL", "grad_rho", "grad_eta_ini"]
@@ -106,6 +111,8 @@ SIGNATURES = {
     "ss_adjoint_download": (I, [P, I, DP, I]),
     "ss_adjoint_step": (I, [P, C.POINTER(ss_params), I]),
     "ss_adjoint_solve": (I, [P, C.POINTER(ss_params), D, D, I, I, D, C.POINTER(ss_adjoint_result)]),
+    "ss_adjoint_solve_mixed": (I, [P, C.POINTER(ss_params), D, D, C.POINTER(ss_mixed_opts),
+                                   C.POINTER(ss_adjoint_mixed_result)]),
     "ss_last_kernel_ms": (I, [P, DP]),
     "ss_pt_iterate_timed": (I, [P, C.POINTER(ss_params), I, DP, DP]),
 }

@@ -184,76 +184,70 @@ __global__ void __launch_bounds__(256) k_adj_prep_edge(AdjArgs a, int fam)
 // ------------------------------------------------------------------ 1. BC fold
 // Threads cover the two other directions; each thread handles both sides of
 // its line (with 3 points in a direction both sides fold onto the same point).
-__global__ void k_adj_bc_ph(AdjArgs a)
+template <typename R> __global__ void k_adj_bc_ph(Box b, int bc_mode, R *VR, R *VT, R *VP)
 {
-    const Box &b = a.b;
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
     const int nr = b.nr, nt = b.nt, np = b.np;
-    const bool cp = a.bc_mode != SS_BC_SHEAR;
-    double *VR = a.in.f[L_VR], *VT = a.in.f[L_VT], *VP = a.in.f[L_VP];
+    const bool cp = bc_mode != SS_BC_SHEAR;
     if (i < nr && j < nt) { // Vph: ghost = c - inner
         const long long lo = b.off(i, j, 0), hi = b.off(i, j, np);
-        VP[lo + 1] -= VP[lo]; VP[lo] = 0.0;
-        VP[hi - 1] -= VP[hi]; VP[hi] = 0.0;
+        VP[lo + 1] -= VP[lo]; VP[lo] = R(0);
+        VP[hi - 1] -= VP[hi]; VP[hi] = R(0);
     }
     if (i < nr && j <= nt) {
         const long long lo = b.off(i, j, 0), hi = b.off(i, j, np - 1);
         if (cp) { VT[lo + 1] += VT[lo]; VT[hi - 1] += VT[hi]; }
-        VT[lo] = 0.0; VT[hi] = 0.0;
+        VT[lo] = R(0); VT[hi] = R(0);
     }
     if (i <= nr && j < nt) {
         const long long lo = b.off(i, j, 0), hi = b.off(i, j, np - 1);
         if (cp) { VR[lo + 1] += VR[lo]; VR[hi - 1] += VR[hi]; }
-        VR[lo] = 0.0; VR[hi] = 0.0;
+        VR[lo] = R(0); VR[hi] = R(0);
     }
 }
 
-__global__ void k_adj_bc_th(AdjArgs a)
+template <typename R> __global__ void k_adj_bc_th(Box b, int bc_mode, R *VR, R *VT, R *VP)
 {
-    const Box &b = a.b;
     const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
     const int nr = b.nr, nt = b.nt, np = b.np;
-    const bool cp = a.bc_mode != SS_BC_SHEAR;
+    const bool cp = bc_mode != SS_BC_SHEAR;
     const long long sj = b.sj;
-    double *VR = a.in.f[L_VR], *VT = a.in.f[L_VT], *VP = a.in.f[L_VP];
     if (i < nr && k <= np) {
         const long long lo = b.off(i, 0, k), hi = b.off(i, nt - 1, k);
         if (cp) { VP[lo + sj] += VP[lo]; VP[hi - sj] += VP[hi]; }
-        VP[lo] = 0.0; VP[hi] = 0.0;
+        VP[lo] = R(0); VP[hi] = R(0);
     }
     if (i < nr && k < np) { // Vth: ghost = c - inner
         const long long lo = b.off(i, 0, k), hi = b.off(i, nt, k);
-        VT[lo + sj] -= VT[lo]; VT[lo] = 0.0;
-        VT[hi - sj] -= VT[hi]; VT[hi] = 0.0;
+        VT[lo + sj] -= VT[lo]; VT[lo] = R(0);
+        VT[hi - sj] -= VT[hi]; VT[hi] = R(0);
     }
     if (i <= nr && k < np) {
         const long long lo = b.off(i, 0, k), hi = b.off(i, nt - 1, k);
         if (cp) { VR[lo + sj] += VR[lo]; VR[hi - sj] += VR[hi]; }
-        VR[lo] = 0.0; VR[hi] = 0.0;
+        VR[lo] = R(0); VR[hi] = R(0);
     }
 }
 
-__global__ void k_adj_bc_R(AdjArgs a)
+template <typename R> __global__ void k_adj_bc_R(Box b, int bc_mode, R *VR, R *VT, R *VP)
 {
-    const Box &b = a.b;
     const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
     const int nr = b.nr, nt = b.nt, np = b.np;
     const long long si = b.si;
-    double *VR = a.in.f[L_VR], *VT = a.in.f[L_VT], *VP = a.in.f[L_VP];
     if (j < nt && k <= np) {
         const long long lo = b.off(0, j, k), hi = b.off(nr - 1, j, k);
-        VP[lo + si] += VP[lo]; VP[lo] = 0.0;
-        VP[hi - si] += VP[hi]; VP[hi] = 0.0;
+        VP[lo + si] += VP[lo]; VP[lo] = R(0);
+        VP[hi - si] += VP[hi]; VP[hi] = R(0);
     }
     if (j <= nt && k < np) {
         const long long lo = b.off(0, j, k), hi = b.off(nr - 1, j, k);
-        VT[lo + si] += VT[lo]; VT[lo] = 0.0;
-        VT[hi - si] += VT[hi]; VT[hi] = 0.0;
+        VT[lo + si] += VT[lo]; VT[lo] = R(0);
+        VT[hi - si] += VT[hi]; VT[hi] = R(0);
     }
     if (j < nt && k < np) { // VR: ghost = -inner
         const long long lo = b.off(0, j, k), hi = b.off(nr, j, k);
-        VR[lo + si] -= VR[lo]; VR[lo] = 0.0;
-        VR[hi - si] -= VR[hi]; VR[hi] = 0.0;
+        VR[lo + si] -= VR[lo]; VR[lo] = R(0);
+        VR[hi - si] -= VR[hi]; VR[hi] = R(0);
     }
 }
 
@@ -648,17 +642,22 @@ __global__ void k_adj_zero(double *p, size_t n)
 // hoist the loads of the argument fields out of the plane loop: each phase
 // loads (with scalar loads) the 20-odd field bases it uses, instead of all
 // 40-odd staying live in scalar registers across the loop and spilling.
-__device__ __forceinline__ const adjf::AArgs &kargs()
+template <typename R> __device__ __forceinline__ const adjf::AArgs<R> &kargs()
 {
     uint64_t v = (uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr();
     asm volatile("" : "+s"(v));
-    return *(const adjf::AArgs *)(const __attribute__((address_space(4))) adjf::AArgs *)v;
+    return *(const adjf::AArgs<R> *)(const __attribute__((address_space(4))) adjf::AArgs<R> *)v;
 }
 
-__global__ void __launch_bounds__(fused::TK * adjf::ATJ) k_adj_fused(adjf::AArgs a, int nx, int ny, int nz)
+// FP32: two workgroups per CU (6 wavefronts per SIMD: at most 80 VGPRs, and
+// 2 x 75 KB of LDS)
+template <typename R> constexpr int ADJ_WAVES_PER_EU = sizeof(R) == 4 ? 6 : 1;
+template <typename R, bool RHS>
+__global__ void __launch_bounds__(fused::TK * adjf::ATJ, ADJ_WAVES_PER_EU<R>)
+    k_adj_fused(adjf::AArgs<R> a, int nx, int ny, int nz)
 {
     using namespace adjf;
-    __shared__ ALds L;
+    __shared__ ALds<R> L;
     const int T_ = nx * ny * nz;
     const int per = (T_ + 7) / 8;
     const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
@@ -667,25 +666,25 @@ __global__ void __launch_bounds__(fused::TK * adjf::ATJ) k_adj_fused(adjf::AArgs
     const int kb = bx * AOK, jb = by * AOJ;
     const int s0 = bz * a.rchunk, s1 = min(s0 + a.rchunk, a.b.nr + 1);
     const int start = max(s0 - 1, 0), stop = min(s1, a.b.nr);
-    AThread T;
+    AThread<R> T;
     athread_init(T, a, threadIdx.x, threadIdx.y, kb, jb);
     aprologue(T, L, a, start);
-    AStepA na;
+    AStepA<R> na;
     aloadA(na, a, T, start);
     __syncthreads(); // the first phase A reads the DT(start) plane the prologue wrote
     for (int q = start; q <= stop; ++q) {
         const bool wr = q >= s0 && q < s1, wr_m = q - 1 >= s0 && q - 1 < s1;
-        const AStepA ca = na;
-        AStepB cb;
-        aloadB(cb, kargs(), T, q);
-        aphaseA(T, L, kargs(), ca, q, wr);
+        const AStepA<R> ca = na;
+        AStepB<R> cb;
+        aloadB(cb, kargs<R>(), T, q);
+        aphaseA<R, RHS>(T, L, kargs<R>(), ca, q, wr);
         __syncthreads();
-        aphaseB(T, L, kargs(), cb, q, wr);
+        aphaseB<R, RHS>(T, L, kargs<R>(), cb, q, wr);
         __syncthreads();
-        aloadA(na, kargs(), T, q + 1); // next step's (clamped into the box at the end)
+        aloadA(na, kargs<R>(), T, q + 1); // next step's (clamped into the box at the end)
         // phase C reads only the phase-B arrays; the next phase A writes only
         // the phase-A arrays, so no barrier is needed between them
-        aphaseC(T, L, kargs(), q, wr, wr_m);
+        aphaseC<R, RHS>(T, L, kargs<R>(), q, wr, wr_m);
     }
 }
 
@@ -693,7 +692,7 @@ __global__ void __launch_bounds__(fused::TK * adjf::ATJ) k_adj_fused(adjf::AArgs
 // the boundary cells copy_BC! fills from each.  Families: 0 -> i in {1,
 // nr-2}; 1 -> j in {1, nt-2} (i inside); 2 -> k in {1, np-2} (i, j inside);
 // each such cell once.
-__global__ void __launch_bounds__(256) k_adj_lb(adjf::AArgs a, double *LB)
+template <typename R> __global__ void __launch_bounds__(256) k_adj_lb(adjf::AArgs<R> a, R *LB)
 {
     const adjf::BoxG &b = a.b;
     const int u = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y * blockDim.y + threadIdx.y;
@@ -714,6 +713,77 @@ __global__ void __launch_bounds__(256) k_adj_lb(adjf::AArgs a, double *LB)
             if (i < 2 || i > nr - 3 || j < 2 || j > nt - 3) continue;
         }
         LB[b.off(i, j, k)] = adjf::lb_sum(a, i, j, k);
+    }
+}
+
+// ------------------------------------------------------------------ FP32 correction
+// (ss_adjoint_solve_mixed) elementwise over whole boxes: the FP32 copy of an
+// FP64 box, the FP32 residual Y - mu, and mu += d in FP64
+__global__ void __launch_bounds__(256) k_adj_to_f32(float *__restrict__ D, const double *__restrict__ S, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        D[q] = (float)S[q];
+}
+__global__ void __launch_bounds__(256) k_adj_resid_f32(float *__restrict__ D, const double *__restrict__ Y,
+                                                       const double *__restrict__ X, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        D[q] = (float)(Y[q] - X[q]);
+}
+__global__ void __launch_bounds__(256) k_adj_acc_f32(double *__restrict__ X, const float *__restrict__ d, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        X[q] = X[q] + (double)d[q];
+}
+__global__ void __launch_bounds__(256) k_adj_zero_f32(float *__restrict__ p, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        p[q] = 0.f;
+}
+__global__ void k_adj_zero_u64(unsigned long long *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// Increment test of the FP32 correction sweeps: max|new - old| (red[0..2])
+// and max|new| (red[3..5]) of lam_VR, lam_Vth, lam_Vph over the faces
+// compute_V! updates (both sets BC-folded), as the bit patterns of
+// non-negative doubles (ordered as integers).
+// ADJ_CHECK_PLANES r-planes per thread, one set of atomics per wavefront.
+constexpr int ADJ_CHECK_PLANES = 32;
+__global__ void __launch_bounds__(256) k_adj_inc_f32(Box b, const float *__restrict__ n0, const float *__restrict__ n1,
+                                                     const float *__restrict__ n2, const float *__restrict__ o0,
+                                                     const float *__restrict__ o1, const float *__restrict__ o2,
+                                                     unsigned long long *red)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    double me[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0};
+    const int i0 = blockIdx.z * ADJ_CHECK_PLANES, i1 = min(i0 + ADJ_CHECK_PLANES, b.nr + 1);
+    for (int i = i0; i < i1 && j <= b.nt && k <= b.np; ++i) {
+        const long long o = b.off(i, j, k);
+        const bool in[3] = {upd_VR(b, i, j, k), upd_VT(b, i, j, k), upd_VP(b, i, j, k)};
+        const float *N[3] = {n0, n1, n2}, *O[3] = {o0, o1, o2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (!in[c]) continue;
+            const double v = N[c][o];
+            me[c] = fmax(me[c], fabs(v - (double)O[c][o]));
+            mv[c] = fmax(mv[c], fabs(v));
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        for (int m = 32; m >= 1; m >>= 1) {
+            me[c] = fmax(me[c], __shfl_xor(me[c], m, 64));
+            mv[c] = fmax(mv[c], __shfl_xor(mv[c], m, 64));
+        }
+    if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (me[c] > 0.0) atomicMax(&red[c], (unsigned long long)__double_as_longlong(me[c]));
+            if (mv[c] > 0.0) atomicMax(&red[3 + c], (unsigned long long)__double_as_longlong(mv[c]));
+        }
     }
 }
 
@@ -799,9 +869,9 @@ AdjArgs make_adj_args(ss_ctx *c, const ss_params &p)
 }
 
 // Arguments of the r-marching kernel for the sets of a (the same step)
-adjf::AArgs fused_args(ss_ctx *c, const AdjArgs &a, bool with_b)
+adjf::AArgs<double> fused_args(ss_ctx *c, const AdjArgs &a, bool with_b)
 {
-    adjf::AArgs f{};
+    adjf::AArgs<double> f{};
     const Box &b = c->box;
     f.b = adjf::BoxG{b.nr, b.nt, b.np, b.sj, b.si};
     f.t = fused::TabsG<double>{c->ftr, c->ftt, c->ft2, c->tab.th, c->tab.ph};
@@ -824,33 +894,49 @@ adjf::AArgs fused_args(ss_ctx *c, const AdjArgs &a, bool with_b)
 // r-chunk length of the marching kernel (march_chunk; 511^3: chunks of 74
 // planes, 12.9 ms, against 13.1-13.6 ms for 26, 37, 47, 57 or 103);
 // SSPT_ADJ_RCHUNK overrides it for measurements.
-int adj_rchunk(const ss_ctx *c, int nplanes, int tiles)
+int adj_rchunk(const ss_ctx *c, int nplanes, int tiles, int per_cu)
 {
     if (const char *e = std::getenv("SSPT_ADJ_RCHUNK")) {
         const int v = std::atoi(e);
         if (v >= 1) return std::min(v, nplanes);
     }
-    return march_chunk(nplanes, tiles, c->n_cu, false);
+    return march_chunk(nplanes, tiles, c->n_cu * per_cu, false);
+}
+
+// The BC fold of lam_V in place (transposes of BC_ph!, BC_th!, BC_R!)
+template <typename R> void launch_adj_fold(ss_ctx *c, int bc_mode, R *VR, R *VT, R *VP)
+{
+    const Box &b = c->box;
+    const int m = std::max(std::max(b.nr, b.nt), b.np) + 1;
+    const dim3 B2(64, 4), G2((m + 63) / 64, (m + 3) / 4);
+    hipLaunchKernelGGL(k_adj_bc_ph<R>, G2, B2, 0, c->stream, b, bc_mode, VR, VT, VP);
+    hipLaunchKernelGGL(k_adj_bc_th<R>, G2, B2, 0, c->stream, b, bc_mode, VR, VT, VP);
+    hipLaunchKernelGGL(k_adj_bc_R<R>, G2, B2, 0, c->stream, b, bc_mode, VR, VT, VP);
+}
+
+// The boundary power-law sums (into LB) and the r-marching kernel for the
+// sets of f (lam_V of f.li BC-folded beforehand)
+template <typename R, bool RHS> void launch_adj_march(ss_ctx *c, adjf::AArgs<R> f, R *LB)
+{
+    const Box &b = c->box;
+    const int m = std::max(std::max(b.nr, b.nt), b.np) + 1;
+    f.LB = LB;
+    hipLaunchKernelGGL(k_adj_lb<R>, dim3((m + 63) / 64, (m + 3) / 4, 3), dim3(64, 4), 0, c->stream, f, LB);
+    const int nx = (b.np + 1 + adjf::AOK - 1) / adjf::AOK, ny = (b.nt + 1 + adjf::AOJ - 1) / adjf::AOJ;
+    f.rchunk = adj_rchunk(c, b.nr + 1, nx * ny, sizeof(R) == 4 ? 2 : 1); // workgroups per CU
+    const int nz = (b.nr + 1 + f.rchunk - 1) / f.rchunk;
+    const int T = nx * ny * nz;
+    hipLaunchKernelGGL((k_adj_fused<R, RHS>), dim3(((T + 7) / 8) * 8), dim3(fused::TK, adjf::ATJ), 0, c->stream, f, nx,
+                       ny, nz);
 }
 
 // lam_out = T^T lam_in (+ bJ on VR if given); gradients when a.grad
 void launch_adj_step(ss_ctx *c, AdjArgs a, bool with_b)
 {
     const Box &b = c->box;
-    const int m = std::max(std::max(b.nr, b.nt), b.np) + 1;
-    const dim3 B2(64, 4), G2((m + 63) / 64, (m + 3) / 4);
-    hipLaunchKernelGGL(k_adj_bc_ph, G2, B2, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_bc_th, G2, B2, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_bc_R, G2, B2, 0, c->stream, a);
+    launch_adj_fold(c, a.bc_mode, a.in.f[L_VR], a.in.f[L_VT], a.in.f[L_VP]);
     if (c->adj_fused) {
-        adjf::AArgs f = fused_args(c, a, with_b);
-        hipLaunchKernelGGL(k_adj_lb, dim3((m + 63) / 64, (m + 3) / 4, 3), B2, 0, c->stream, f, c->adj[AD_LB]);
-        const int nx = (b.np + 1 + adjf::AOK - 1) / adjf::AOK, ny = (b.nt + 1 + adjf::AOJ - 1) / adjf::AOJ;
-        f.rchunk = adj_rchunk(c, b.nr + 1, nx * ny);
-        const int nz = (b.nr + 1 + f.rchunk - 1) / f.rchunk;
-        const int T = nx * ny * nz;
-        hipLaunchKernelGGL(k_adj_fused, dim3(((T + 7) / 8) * 8), dim3(fused::TK, adjf::ATJ), 0, c->stream, f, nx, ny,
-                           nz);
+        launch_adj_march<double, false>(c, fused_args(c, a, with_b), c->adj[AD_LB]);
         if (a.grad) {
             hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
             hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
@@ -874,6 +960,81 @@ int adj_field(int field, int *fwd, int *box)
     *fwd = map[field];
     *box = field < L_N ? -1 : (field == SS_ADJ_GRAD_RHO ? AD_GRHO : AD_GINI);
     return SS_OK;
+}
+
+// Gradients from the converged lam (set adj_cur): one more backward pass on
+// a BC-folded copy in the other set (its lam outputs are not needed)
+int adj_gradients(ss_ctx *c, const ss_params &p)
+{
+    const Box &b = c->box;
+    // gradients from the converged lam: one more backward pass (its outputs are discarded)
+    AdjArgs a = make_adj_args(c, p);
+    a.grad = 1;
+    hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GRHO], c->box_elems);
+    hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GINI], c->box_elems);
+    // keep lam: fold and differentiate a copy in the other set
+    AdjSet cur = set_of(c, c->adj_cur), oth = set_of(c, 1 - c->adj_cur);
+    for (int q = 0; q < L_N; ++q)
+        if (hipMemcpyAsync(oth.f[q], cur.f[q], c->box_elems * sizeof(double), hipMemcpyDeviceToDevice, c->stream) !=
+            hipSuccess)
+            return afail(c, SS_ERR_STATE, "adjoint gradient copy");
+    a.in = oth;
+    a.out = oth;
+    launch_adj_fold(c, a.bc_mode, a.in.f[L_VR], a.in.f[L_VT], a.in.f[L_VP]);
+    hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
+    return sync_status(c, "adjoint gradients");
+}
+
+// FP32 boxes of the mixed-precision adjoint solve
+enum { AM_D = 0, AM_RHS = 2 * L_N, AM_ST = 3 * L_N, AM_LB = 3 * L_N + 12, AM_NBOX = 3 * L_N + 13 };
+static_assert(AM_NBOX == SS_ADJ32_NBOX, "FP32 adjoint box count");
+
+int ensure_adj32(ss_ctx *c)
+{
+    for (int q = 0; q < AM_NBOX; ++q) {
+        if (c->adj32[q]) continue;
+        if (hipMalloc(&c->adj32[q], c->box_elems * sizeof(float)) != hipSuccess)
+            return afail(c, SS_ERR_NOMEM, "FP32 adjoint boxes");
+        hipLaunchKernelGGL(k_adj_zero_f32, dim3(2048), dim3(256), 0, c->stream, c->adj32[q], c->box_elems);
+    }
+    return SS_OK;
+}
+
+// Arguments of the FP32 correction sweeps: the FP32 state copies, the FP32
+// tables, right-hand side set AM_RHS (lam sets filled in per sweep)
+adjf::AArgs<float> f32_args(ss_ctx *c, const AdjArgs &a)
+{
+    adjf::AArgs<float> f{};
+    const Box &b = c->box;
+    float *const *F = c->adj32;
+    f.b = adjf::BoxG{b.nr, b.nt, b.np, b.sj, b.si};
+    f.t = fused::TabsG<float>{c->ftr32, c->ftt32, c->ft232, c->th32, c->ph32};
+    for (int q = 0; q < L_N; ++q) f.rhs[q] = F[AM_RHS + q];
+    const float *const *S = F + AM_ST;
+    f.tRR = S[0]; f.tTT = S[1]; f.tPP = S[2]; f.tRT = S[3]; f.tRP = S[4]; f.tTP = S[5];
+    f.eta = S[6]; f.ini = S[7]; f.ePL = S[8]; f.tII = S[9]; f.DT = S[10]; f.SM = S[11];
+    f.bJ = nullptr;
+    f._dr = (float)a._dr; f._dth = (float)a._dth; f._dph = (float)a._dph;
+    f.kb = (float)a.kb; f.gs = (float)a.gs; f.tC = (float)a.tC; f.ex = (float)a.ex;
+    f.relax = (float)a.relax; f.emin = (float)a.emin; f.emax = (float)a.emax;
+    f.do_relax = a.do_relax; f.do_lo = a.do_lo; f.do_hi = a.do_hi;
+    return f;
+}
+
+// max over the velocity components of red[c] / red[3 + c] (bit patterns of
+// non-negative doubles); -1 when a NaN was seen
+double ratio_of(const unsigned long long *red)
+{
+    double err = 0.0;
+    for (int q = 0; q < 3; ++q) {
+        double dv, vv;
+        memcpy(&dv, &red[q], 8);
+        memcpy(&vv, &red[3 + q], 8);
+        if (std::isnan(dv) || std::isnan(vv)) return -1.0;
+        if (vv > 0.0) err = std::fmax(err, dv / vv);
+    }
+    return err;
 }
 
 using aclk = std::chrono::steady_clock;
@@ -1002,7 +1163,6 @@ int ss_adjoint_solve(ss_ctx *c, const ss_params *p, double r, double radius, int
     if (rc != SS_OK) return rc;
     if ((rc = ss_adjoint_zero(c)) != SS_OK) return rc;
     if ((rc = ss_adjoint_objective(c, r, radius, &out->J)) != SS_OK) return rc;
-    const Box &b = c->box;
     int it = 0;
     while (it < maxiter) {
         const bool check = ((it + 1) % ncheck) == 0;
@@ -1040,27 +1200,114 @@ int ss_adjoint_solve(ss_ctx *c, const ss_params *p, double r, double radius, int
         }
     }
     out->iters = it;
-    // gradients from the converged lam: one more backward pass (its outputs are discarded)
-    AdjArgs a = make_adj_args(c, *p);
-    a.grad = 1;
-    hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GRHO], c->box_elems);
-    hipLaunchKernelGGL(k_adj_zero, dim3(2048), dim3(256), 0, c->stream, c->adj[AD_GINI], c->box_elems);
-    // keep lam: fold and differentiate a copy in the other set
-    AdjSet cur = set_of(c, c->adj_cur), oth = set_of(c, 1 - c->adj_cur);
-    for (int q = 0; q < L_N; ++q)
-        if (hipMemcpyAsync(oth.f[q], cur.f[q], c->box_elems * sizeof(double), hipMemcpyDeviceToDevice, c->stream) !=
+    if ((rc = adj_gradients(c, *p)) != SS_OK) return rc;
+    out->seconds = std::chrono::duration<double>(aclk::now() - t0).count();
+    return out->converged ? SS_OK : SS_NOT_CONVERGED;
+}
+
+int ss_adjoint_solve_mixed(ss_ctx *c, const ss_params *p, double r, double radius, const ss_mixed_opts *o,
+                           ss_adjoint_mixed_result *out)
+{
+    if (!c || !p || !o || !out) return SS_ERR_ARG;
+    if (o->max_outer < 1 || o->max_outer > SS_MIXED_MAX_OUTER || o->max_inner < 1 || o->nout_inner < 1)
+        return afail(c, SS_ERR_ARG, "bad mixed-solve options");
+    memset(out, 0, sizeof *out);
+    out->err = NAN;
+    const aclk::time_point t0 = aclk::now();
+    int rc = ss_adjoint_prepare(c, p);
+    if (rc != SS_OK) return rc;
+    if ((rc = ss_adjoint_zero(c)) != SS_OK) return rc;
+    if ((rc = ss_adjoint_objective(c, r, radius, &out->J)) != SS_OK) return rc;
+    if ((rc = ensure_adj32(c)) != SS_OK) return rc;
+    const size_t n = c->box_elems;
+    const dim3 FG(2048), FB(256);
+    float *const *F = c->adj32;
+    AdjArgs a0 = make_adj_args(c, *p);
+    {   // the converged state in FP32, once
+        const double *st[12] = {a0.tRR, a0.tTT, a0.tPP, a0.tRT, a0.tRP, a0.tTP,
+                                a0.eta, a0.ini, a0.ePL, a0.tII, a0.DT, a0.SM};
+        for (int q = 0; q < 12; ++q) hipLaunchKernelGGL(k_adj_to_f32, FG, FB, 0, c->stream, F[AM_ST + q], st[q], n);
+    }
+    adjf::AArgs<float> f = f32_args(c, a0);
+    const Box &b = c->box;
+    const dim3 CB(64, 4), CG((b.np + 1 + 63) / 64, (b.nt + 1 + 3) / 4, (b.nr + 1 + ADJ_CHECK_PLANES - 1) / ADJ_CHECK_PLANES);
+    double tin_prev = o->tol_inner, err_prev = NAN;
+    for (int outer = 0; outer < o->max_outer; ++outer) {
+        // Y = F(T^T mu + dJ/dx) into the other set; err = max|Y - mu| / max|Y|
+        AdjArgs a = make_adj_args(c, *p);
+        launch_adj_step(c, a, true);
+        launch_adj_fold(c, p->bc_mode, a.out.f[L_VR], a.out.f[L_VT], a.out.f[L_VP]);
+        FieldSet y{};
+        y.f[SS_VR] = a.out.f[L_VR];
+        y.f[SS_VTH] = a.out.f[L_VT];
+        y.f[SS_VPH] = a.out.f[L_VP];
+        launch_check_err(c, y, a.in.f[L_VR], a.in.f[L_VT], a.in.f[L_VP], 0);
+        if (hipMemcpyAsync(c->red_host, c->red, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream) !=
             hipSuccess)
-            return afail(c, SS_ERR_STATE, "adjoint gradient copy");
-    a.in = oth;
-    a.out = oth;
-    const int m = std::max(std::max(b.nr, b.nt), b.np) + 1;
-    const dim3 B2(64, 4), G2((m + 63) / 64, (m + 3) / 4);
-    hipLaunchKernelGGL(k_adj_bc_ph, G2, B2, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_bc_th, G2, B2, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_bc_R, G2, B2, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_power_law, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
-    hipLaunchKernelGGL(k_adj_grad_rho, agrid(b.nr, b.nt, b.np), AB, 0, c->stream, a);
-    if ((rc = sync_status(c, "adjoint gradients")) != SS_OK) return rc;
+            return afail(c, SS_ERR_STATE, "adjoint check readback");
+        if ((rc = sync_status(c, "adjoint outer step")) != SS_OK) return rc;
+        const double err = ratio_of(c->red_host);
+        if (err < 0.0) return afail(c, SS_ERR_NONFINITE, "adjoint NaN");
+        out->err = err;
+        out->outer = outer;
+        if (err < o->tol) { // Y is the iterate
+            c->adj_cur = 1 - c->adj_cur;
+            out->converged = 1;
+            break;
+        }
+        // The reduction of an outer step scales with the inner tolerance it
+        // used; when the reduction still needed is smaller than the last
+        // step's, the inner tolerance is loosened in proportion (factor 2 of
+        // margin, at most 1e-2), never tightened beyond o->tol_inner.
+        double tin = o->tol_inner;
+        if (std::isfinite(err_prev) && err_prev > 0.0 && err < err_prev) {
+            const double q = err / err_prev, want = 0.5 * (o->tol / err);
+            if (want > q) tin = std::fmin(1e-2, tin_prev * (want / q));
+            tin = std::fmax(tin, o->tol_inner);
+        }
+        err_prev = err;
+        tin_prev = tin;
+        // right-hand side Y - mu in FP32; both correction sets zero
+        for (int q = 0; q < L_N; ++q)
+            hipLaunchKernelGGL(k_adj_resid_f32, FG, FB, 0, c->stream, F[AM_RHS + q], a.out.f[q], a.in.f[q], n);
+        for (int q = 0; q < 2 * L_N; ++q) hipLaunchKernelGGL(k_adj_zero_f32, FG, FB, 0, c->stream, F[AM_D + q], n);
+        const aclk::time_point ti0 = aclk::now();
+        int cur = 0, it;
+        for (it = 1; it <= o->max_inner; ++it) {
+            float *const *in = F + AM_D + L_N * cur, *const *nw = F + AM_D + L_N * (1 - cur);
+            for (int q = 0; q < L_N; ++q) {
+                f.li[q] = in[q];
+                f.lo[q] = nw[q];
+            }
+            launch_adj_fold(c, p->bc_mode, in[L_VR], in[L_VT], in[L_VP]);
+            launch_adj_march<float, true>(c, f, F[AM_LB]);
+            cur = 1 - cur;
+            if (it % o->nout_inner == 0) {
+                // compare folded with folded: the new set's own fold at the
+                // next sweep then changes nothing (the fold is idempotent)
+                launch_adj_fold(c, p->bc_mode, nw[L_VR], nw[L_VT], nw[L_VP]);
+                hipLaunchKernelGGL(k_adj_zero_u64, dim3(1), dim3(64), 0, c->stream, c->red, 6);
+                hipLaunchKernelGGL(k_adj_inc_f32, CG, CB, 0, c->stream, b, nw[L_VR], nw[L_VT], nw[L_VP], in[L_VR],
+                                   in[L_VT], in[L_VP], c->red);
+                if (hipMemcpyAsync(c->red_host, c->red, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   c->stream) != hipSuccess)
+                    return afail(c, SS_ERR_STATE, "adjoint inner readback");
+                if ((rc = sync_status(c, "adjoint FP32 sweeps")) != SS_OK) return rc;
+                const double ein = ratio_of(c->red_host);
+                if (ein < 0.0) return afail(c, SS_ERR_NONFINITE, "adjoint FP32 sweep NaN");
+                if (ein < tin) break;
+            }
+        }
+        if ((rc = sync_status(c, "adjoint FP32 sweeps")) != SS_OK) return rc;
+        out->inner_seconds += std::chrono::duration<double>(aclk::now() - ti0).count();
+        out->inner_total += std::min(it, o->max_inner);
+        // mu += F d (FP64)
+        float *const *d = F + AM_D + L_N * cur;
+        launch_adj_fold(c, p->bc_mode, d[L_VR], d[L_VT], d[L_VP]);
+        for (int q = 0; q < L_N; ++q) hipLaunchKernelGGL(k_adj_acc_f32, FG, FB, 0, c->stream, a.in.f[q], d[q], n);
+        out->outer = outer + 1;
+    }
+    if ((rc = adj_gradients(c, *p)) != SS_OK) return rc;
     out->seconds = std::chrono::duration<double>(aclk::now() - t0).count();
     return out->converged ? SS_OK : SS_NOT_CONVERGED;
 }

@@ -244,6 +244,8 @@ int ss_destroy(ss_ctx *c)
         if (p) (void)hipFree(p);
     for (double *p : c->adj)
         if (p) (void)hipFree(p);
+    for (float *p : c->adj32)
+        if (p) (void)hipFree(p);
     if (c->adj_part) (void)hipFree(c->adj_part);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

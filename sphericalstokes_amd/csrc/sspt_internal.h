@@ -75,7 +75,7 @@ enum { SS_F32_ETA = 20, SS_F32_GP = 21, SS_F32_FVR = 22, SS_F32_FVT = 23, SS_F32
        SS_F32_NBOX = 27 };
 
 // FP64 boxes of the adjoint solve (sspt_adjoint.hip lists them)
-enum { SS_ADJ_NBOX = 50 };
+enum { SS_ADJ_NBOX = 50, SS_ADJ32_NBOX = 49 };
 
 // Length of the r-chunks of a marching kernel (grid z).  A workgroup marches
 // its chunk plus two warm-up planes, and one workgroup fits a CU at a time,
@@ -135,6 +135,7 @@ struct ss_ctx {
     bool adj_ready = false;
     int adj_cur = 0;                // lam set holding the current iterate
     double adj_obj_r = 0.0, adj_obj_rad = 0.0; // inclusion of the objective (ss_adjoint_objective)
+    float *adj32[sspt::SS_ADJ32_NBOX] = {}; // FP32 adjoint correction: two sets, right-hand side, state, LB
     bool adj_fused = true;          // r-marching adjoint kernel, else the per-family kernels (set by ss_adjoint_prepare)
     bool grid_set = false;
     double th_lo = 0.0, th_hi = 0.0; // th[0], th[nth-1] (host copies for the pure-shear BC)

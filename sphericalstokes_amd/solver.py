@@ -304,3 +304,23 @@ class Solver:
         return {"J": res.J, "iters": res.iters, "converged": bool(res.converged), "err": res.err,
                 "seconds": res.seconds, "grad_rho": self.adjoint_download("grad_rho"),
                 "grad_log10_eta_ini": g_ini * self.download("eta_ini") * math.log(10.0)}
+
+    def adjoint_solve_mixed(self, p: Params, r: float, radius: float, *, max_outer=20, max_inner=200_000,
+                            nout_inner=100, tol_inner=1e-4, tol=1e-12, gradients=True):
+        """adjoint_solve with FP32 sweeps inside an FP64 defect correction
+        (ss_adjoint_solve_mixed): dict with J, outer steps, FP32 sweeps and,
+        with gradients, the gradient arrays as adjoint_solve returns them."""
+        sp = ss_params.from_params(p)
+        o = ss_mixed_opts(int(max_outer), int(max_inner), int(nout_inner), float(tol_inner), float(tol))
+        res = _lib.ss_adjoint_mixed_result()
+        rc = self._L.ss_adjoint_solve_mixed(self._h, C.byref(sp), float(r), float(radius), C.byref(o),
+                                            C.byref(res))
+        if rc not in (0, _lib.SS_NOT_CONVERGED):
+            self._chk(rc, "adjoint_solve_mixed")
+        out = {"J": res.J, "outer": res.outer, "inner_total": res.inner_total, "converged": bool(res.converged),
+               "err": res.err, "seconds": res.seconds, "inner_seconds": res.inner_seconds}
+        if gradients:
+            g_ini = self.adjoint_download("grad_eta_ini")
+            out["grad_rho"] = self.adjoint_download("grad_rho")
+            out["grad_log10_eta_ini"] = g_ini * self.download("eta_ini") * math.log(10.0)
+        return out

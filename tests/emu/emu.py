@@ -142,6 +142,37 @@ def adj_step(setup, state, lam, params, rchunk=0, objective=None):
     return out
 
 
+
+def adj_step_rhs(setup, state, lam, params, rhs, f32=False, rchunk=0):
+    """lam_out = T^T lam + rhs (every output) with the RHS instances of the
+    r-marching adjoint kernel on the host, in FP64 or (f32) with state, lam,
+    rhs and the tables rounded to FP32 and FP32 arithmetic."""
+    L = lib()
+    VP = C.c_void_p
+    DP = C.POINTER(C.c_double)
+    if not hasattr(L, "_adj_rhs_sig"):
+        L.emu_adj_step_rhs.restype = C.c_int
+        L.emu_adj_step_rhs.argtypes = [C.c_int] * 5 + [DP] * 3 + [C.POINTER(ss_params), C.POINTER(VP), C.POINTER(VP),
+                                                                  C.POINTER(VP), C.c_int, C.POINTER(VP)]
+        L._adj_rhs_sig = True
+    dt = np.float32 if f32 else np.float64
+    sb = [_box1(setup, n, state[n]).astype(dt) for n in ADJ_STATE]
+    li = [_box1(setup, n, lam[n]).astype(dt) for n in ADJ]
+    rb = [_box1(setup, n, rhs[n]).astype(dt) for n in ADJ]
+    lo = [np.zeros_like(x) for x in li]
+    arr = lambda xs: (VP * len(xs))(*[x.ctypes.data for x in xs])
+    sp = ss_params.from_params(params)
+    R, th, ph = (np.ascontiguousarray(x, dtype=np.float64) for x in (setup.R, setup.th, setup.ph))
+    L.emu_adj_step_rhs(int(f32), setup.nr, setup.nth, setup.nph, pitch_of(setup.nph), R.ctypes.data_as(DP),
+                       th.ctypes.data_as(DP), ph.ctypes.data_as(DP), C.byref(sp), arr(sb), arr(li), arr(lo),
+                       int(rchunk), arr(rb))
+    out = {}
+    for n, x in zip(ADJ, lo):
+        o = 1 if n in SHEAR else 0
+        s = lam[n].shape
+        out[n] = np.asfortranarray(x[o:o + s[0], o:o + s[1], o:o + s[2]].astype(np.float64))
+    return out
+
 def pow_ex(x, ex):
     """The fused kernels' x^ex (sspt_fused_body.h pow_ex), run on the host."""
     x = np.ascontiguousarray(x, dtype=np.float64)

@@ -71,3 +71,37 @@ def test_marching_adjoint_objective_matches_reference(shape):
         scale = max(np.abs(ref[n]).max(), 1e-300)
         err = np.abs(got[n] - ref[n]).max() / scale
         assert err <= 1e-12, (n, err)
+
+
+@pytest.mark.parametrize("shape", [(11, 13, 70, 0), (9, 30, 7, 2), (5, 5, 130, 1)],
+                         ids=["11x13x70_r0", "9x30x7_r2", "5x5x130_r1"])
+@pytest.mark.parametrize("case", list(CASES))
+def test_marching_adjoint_rhs_instances(case, shape):
+    """The right-hand-side instances of the kernel (the correction sweeps
+    d = T^T d + r of the mixed-precision adjoint solve): in FP64 the output
+    is the plain step's plus r at every point of every field, bit for bit
+    (the kernel writes each point of each field once, and adding r is its
+    last operation); in FP32 (state, lam, r and tables rounded, FP32
+    arithmetic) within 2e-6 of each field's max of the oracle's T^T lam + r
+    (FP32 rounding of a few dozen terms: 8e-8 to 3.3e-7 measured)."""
+    nr, nth, nph, rch = shape
+    st = setups.diapir_sector(nr, nth=nth, nph=nph, **CASES[case])
+    setups.random_state(st, seed=11)
+    o = Oracle(st)
+    o.step(2)
+    rng = np.random.default_rng(5)
+    lam = {n: rng.standard_normal(o.f[n].shape) for n in ORC_ADJ}
+    rhs = {n: rng.standard_normal(o.f[n].shape) for n in ORC_ADJ}
+    if st.params.relax == 1.0:
+        lam["eta_PL"][:] = 0.0
+        rhs["eta_PL"][:] = 0.0   # eta_PL is not an output then
+    copy = lambda d: {k: v.copy() for k, v in d.items()}
+    base = emu.adj_step(st, o.f, copy(lam), st.params, rchunk=rch)
+    got = emu.adj_step_rhs(st, o.f, copy(lam), st.params, rhs, rchunk=rch)
+    g32 = emu.adj_step_rhs(st, o.f, copy(lam), st.params, rhs, f32=True, rchunk=rch)
+    ref = o.adj_step(copy(lam))
+    for n in ORC_ADJ:
+        assert np.array_equal(got[n], base[n] + rhs[n]), n
+        e = ref[n] + rhs[n]
+        rel = np.abs(g32[n] - e).max() / max(np.abs(e).max(), 1e-300)
+        assert rel <= 2e-6, (n, rel)
