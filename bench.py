@@ -98,6 +98,17 @@ def adjoint_iteration_ms(s, p, steps, r, radius):
     return (time.perf_counter() - t0) / steps * 1e3
 
 
+def adjoint_fp32_sweep_ms(s, p, sweeps, r, radius):
+    """Average wall time of one FP32 correction sweep of the mixed-precision
+    adjoint solve (BC fold, boundary power-law sums and the FP32 r-marching
+    kernel with its right-hand side) at the state held by s: one outer step
+    of ss_adjoint_solve_mixed with tol 0 and no inner checks, from the
+    library's own timing of its sweeps."""
+    m = s.adjoint_solve_mixed(p, r, radius, max_outer=1, max_inner=sweeps + 1, nout_inner=sweeps + 2,
+                              tol_inner=0.0, tol=0.0, gradients=False)
+    return m["inner_seconds"] / m["inner_total"] * 1e3
+
+
 def adjoint_solve_ratio(n, eps, mixed=True):
     """Forward solve (reference exit test, relative change of V < eps every
     500 iterations) and adjoint solve (relative change of the adjoint < eps)
@@ -428,6 +439,8 @@ def main():
                     help="adjoint iterations timed on the main workload; 0 skips them")
     ap.add_argument("--adjoint-n", type=int, default=63,
                     help="size of the forward-vs-adjoint solve comparison; 0 skips it")
+    ap.add_argument("--adjoint-eps", type=float, default=1e-10,
+                    help="exit tolerance of the forward and adjoint solves of that comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain field copy (k_copy_field: n^3 doubles read + written) so that "
                          "rocprofv3 --pmc byte counters can be scaled")
@@ -464,6 +477,7 @@ def main():
     sweep_ms = fp32_sweep_ms(s, p, args.fp32_sweeps) if args.fp32_sweeps > 0 else None
     # adjoint iteration on the same state (J: mean VR in a sphere of radius 0.01 in the crust)
     adj_ms = adjoint_iteration_ms(s, p, args.adjoint_steps, 0.985, 0.01 ** 2) if args.adjoint_steps > 0 else None
+    adj32_ms = adjoint_fp32_sweep_ms(s, p, args.adjoint_steps, 0.985, 0.01 ** 2) if args.adjoint_steps > 0 else None
     s.close()
     ms_per_step = wall / args.steps * 1e3
     A = a_eff_bytes(n)
@@ -499,7 +513,9 @@ def main():
     }
     out["adjoint"] = {"iter_ms": round(adj_ms, 4) if adj_ms else None,
                       "iter_over_fp64_iter": round(adj_ms / ms_per_step, 4) if adj_ms else None,
-                      "solve": adjoint_solve_ratio(args.adjoint_n, 1e-10) if args.adjoint_n > 0 else None}
+                      "fp32_sweep_ms": round(adj32_ms, 4) if adj32_ms else None,
+                      "fp32_sweep_over_fp64_iter": round(adj32_ms / ms_per_step, 4) if adj32_ms else None,
+                      "solve": adjoint_solve_ratio(args.adjoint_n, args.adjoint_eps) if args.adjoint_n > 0 else None}
     if args.ttt_n > 0:
         out["mixed_precision"]["time_to_tol"] = time_to_tolerance(args.ttt_n, args.ttt_tol, args.ttt_tol_inner,
                                                                   args.ttt_setup)

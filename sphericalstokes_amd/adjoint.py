@@ -36,6 +36,8 @@ class SolveOptions:
     adj_maxiter: int = 4_000_000
     adj_ncheck: int = 200
     adj_tol: float = 1e-13
+    adj_mixed: bool = False      # FP32 sweeps inside an FP64 defect correction (ss_adjoint_solve_mixed)
+    adj_max_outer: int = 20       # its outer steps
     strict: bool = True          # raise NotConvergedError on an unconverged forward or adjoint solve
     info: dict = field(default_factory=dict)
 
@@ -88,7 +90,12 @@ class AscentVelocity(torch.autograd.Function):
     def backward(ctx, gJ):
         s, setup, o = ctx.solver, ctx.setup, ctx.opts
         try:
-            r = s.adjoint_solve(setup.params, o.r, o.radius, o.adj_maxiter, o.adj_ncheck, o.adj_tol)
+            if o.adj_mixed:
+                r = s.adjoint_solve_mixed(setup.params, o.r, o.radius, max_outer=o.adj_max_outer,
+                                          max_inner=o.adj_maxiter, nout_inner=o.adj_ncheck, tol=o.adj_tol)
+                r["iters"] = r["inner_total"]
+            else:
+                r = s.adjoint_solve(setup.params, o.r, o.radius, o.adj_maxiter, o.adj_ncheck, o.adj_tol)
         finally:
             s.close()
             ctx.solver = None

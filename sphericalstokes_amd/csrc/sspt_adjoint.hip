@@ -1271,6 +1271,7 @@ int ss_adjoint_solve_mixed(ss_ctx *c, const ss_params *p, double r, double radiu
         for (int q = 0; q < L_N; ++q)
             hipLaunchKernelGGL(k_adj_resid_f32, FG, FB, 0, c->stream, F[AM_RHS + q], a.out.f[q], a.in.f[q], n);
         for (int q = 0; q < 2 * L_N; ++q) hipLaunchKernelGGL(k_adj_zero_f32, FG, FB, 0, c->stream, F[AM_D + q], n);
+        if ((rc = sync_status(c, "adjoint correction set-up")) != SS_OK) return rc;
         const aclk::time_point ti0 = aclk::now();
         int cur = 0, it;
         for (it = 1; it <= o->max_inner; ++it) {

@@ -103,3 +103,24 @@ def test_mixed_adjoint_option_errors():
         o = _lib.ss_mixed_opts(4, 10, 10, 1e-4, 1e-8)
         assert L.ss_adjoint_solve_mixed(s._h, None, R0, RAD, C.byref(o), C.byref(res)) == SS_ERR_ARG
         assert L.ss_adjoint_solve_mixed(s._h, C.byref(sp), R0, RAD, C.byref(o), None) == SS_ERR_ARG
+
+
+def test_autograd_function_mixed_adjoint():
+    """The torch.autograd.Function with the mixed-precision backward
+    (SolveOptions.adj_mixed) gives the FP64 backward's gradients."""
+    import torch
+    from sphericalstokes_amd.adjoint import SolveOptions, ascent_velocity
+
+    st = setups.diapir_sector(13, n_exp=3.0, bc_mode=BC_FREE_SLIP, vr=10.0)
+    rho0 = np.ascontiguousarray(st.fields["rho"])
+    le0 = np.log10(np.ascontiguousarray(st.fields["eta_ini"]))
+    grads = {}
+    for mixed in (False, True):
+        opts = SolveOptions(eps=1e-13, nout=500, adj_mixed=mixed, adj_ncheck=100)
+        rho = torch.tensor(rho0, requires_grad=True)
+        le = torch.tensor(le0, requires_grad=True)
+        ascent_velocity(rho, le, st, opts).backward()
+        say("autograd, mixed" if mixed else "autograd, fp64", opts.info.get("adjoint"))
+        grads[mixed] = (rho.grad.numpy(), le.grad.numpy())
+    for a, b in zip(grads[True], grads[False]):
+        assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
