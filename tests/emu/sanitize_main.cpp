@@ -51,6 +51,9 @@ extern "C" int emu_fused_iterations(int nr, int nt, int np, int pitch, const dou
 extern "C" int emu_adj_step(int nr, int nt, int np, int pitch, const double *R1, const double *th1, const double *ph1,
                             const ss_params *p, double **state, double **lam_in, double **lam_out, int rchunk,
                             const double *bJ, double obj_r, double obj_rad);
+extern "C" int emu_adj_step_rhs(int f32, int nr, int nt, int np, int pitch, const double *R1, const double *th1,
+                                const double *ph1, const ss_params *p, void **state, void **lam_in, void **lam_out,
+                                int rchunk, void **rhs);
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -96,6 +99,17 @@ double *alloc_region(size_t n)
     return p;
 }
 void free_region(double *p)
+{
+    g_regions.erase(reinterpret_cast<const char *>(p));
+    delete[] p;
+}
+float *alloc_region_f(size_t n)
+{
+    float *p = new float[n]();
+    g_regions[reinterpret_cast<const char *>(p)] = n * sizeof(float);
+    return p;
+}
+void free_region_f(float *p)
 {
     g_regions.erase(reinterpret_cast<const char *>(p));
     delete[] p;
@@ -267,6 +281,51 @@ int run_case(const Case &c)
             std::fprintf(stderr, "adjoint mismatch: case %dx%dx%d field %d err %.3e of %.3e\n", nr, nt, np, q, num, mx);
             ++bad;
         }
+    }
+    // the FP32 right-hand-side instance on the same case: rhs = dJ on VR and 0
+    // elsewhere, so T^T lam + rhs is lam_ref again (to FP32 rounding)
+    {
+        std::vector<float *> owned3;
+        auto to_box_f = [&](const std::vector<double> &src, int f) {
+            float *b = alloc_region_f(box);
+            owned3.push_back(b);
+            const Shape s = shape_of(f, nr, nt, np);
+            for (int k = 0; k < s.n3; ++k)
+                for (int j = 0; j < s.n2; ++j)
+                    for (int i = 0; i < s.n1; ++i)
+                        b[(size_t)(i + s.off) * (nt + 1) * pitch + (size_t)(j + s.off) * pitch + (k + s.off)] =
+                            (float)src[((size_t)k * s.n2 + j) * s.n1 + i];
+            return b;
+        };
+        void *sf[12], *lif[NA], *lof[NA], *rf[NA];
+        for (int q = 0; q < 12; ++q) sf[q] = to_box_f(O[st_field[q]], st_field[q]);
+        for (int q = 0; q < NA; ++q) {
+            lif[q] = to_box_f(lam[q], adj_field[q]);
+            lof[q] = alloc_region_f(box);
+            owned3.push_back((float *)lof[q]);
+            rf[q] = q == ORC_A_VR ? to_box_f(dJ, SS_VR) : alloc_region_f(box);
+            if (q != ORC_A_VR) owned3.push_back((float *)rf[q]);
+        }
+        emu_adj_step_rhs(1, nr, nt, np, pitch, R.data(), th.data(), ph.data(), &p, sf, lif, lof, c.rchunk, rf);
+        for (int q = 0; q < NA; ++q) {
+            const Shape s = shape_of(adj_field[q], nr, nt, np);
+            const float *lo32 = (const float *)lof[q];
+            double num = 0.0, mx = 0.0;
+            for (int k = 0; k < s.n3; ++k)
+                for (int j = 0; j < s.n2; ++j)
+                    for (int i = 0; i < s.n1; ++i) {
+                        const double a = lo32[(size_t)(i + s.off) * (nt + 1) * pitch + (size_t)(j + s.off) * pitch + (k + s.off)];
+                        const double b = lam_ref[q][((size_t)k * s.n2 + j) * s.n1 + i];
+                        num = std::fmax(num, std::fabs(a - b));
+                        mx = std::fmax(mx, std::fabs(b));
+                    }
+            if (!(num <= 2e-6 * std::fmax(mx, 1e-300))) {
+                std::fprintf(stderr, "FP32 adjoint mismatch: case %dx%dx%d field %d err %.3e of %.3e\n", nr, nt, np, q,
+                             num, mx);
+                ++bad;
+            }
+        }
+        for (float *b : owned3) free_region_f(b);
     }
     for (double *b : owned) free_region(b);
     for (double *b : owned2) free_region(b);

@@ -2,7 +2,8 @@
 r-marching adjoint kernels built with -fsanitize=address,undefined and run
 over the fused kernels' tiling-edge shapes (tests/emu/sanitize_main.cpp):
 no out-of-bounds access, no undefined behaviour, and the emulated
-iterations and adjoint step agree with the oracle.  Every field access
+iterations and adjoint step (FP64, and the FP32 right-hand-side instance
+of the mixed-precision adjoint) agree with the oracle.  Every field access
 through the kernels' 32-bit byte offsets is also checked against the box
 allocation it is based on."""
 import os
