@@ -7,9 +7,9 @@
   record, ascent velocity and a SHA-256 of every field); the solve to an
   FP64 residual of 1e-8 with physical checks of the result.
 * Power-law diapir under far-field pure shear, 255^3 (n_exp = 3, viscosity
-  cutoffs 1e-4..1e4, relaxation 0.5): every field after 2000 fixed fused
-  iterations within rel-L2 1e-10 of the oracle, with the power law active
-  (finite eta_PL, eta != 2 eta_ini).
+  cutoffs 1e-4..1e4, relaxation 0.5): every field after 1000 and after 2000
+  fixed fused iterations within rel-L2 1e-10 of the oracle, with the power
+  law active (finite eta_PL, eta != 2 eta_ini).
 * Plateau, 511^3 (power-law crust, free slip): 3 fused iterations against
   the oracle over the full arrays.
 """
@@ -128,17 +128,40 @@ def test_config1_63_residual_solve():
 
 
 # ---------------------------------------------------------------- 255^3 power-law pure shear
-def test_config2_255_powerlaw_pure_shear_2000_iterations():
-    n, iters, chunk = 255, 2000, 100
-    st = setups.diapir_sector(n, n_exp=3.0, bc_mode=BC_PURE_SHEAR, sh=1.0, relax=0.5, eta_min=1e-4, eta_max=1e4)
-    say("255^3 power-law diapir, pure shear: 2000 fused iterations vs the oracle")
-    o = Oracle(st)
-    s = _solver(st)
+@pytest.fixture(scope="module")
+def config2_run():
+    """The 255^3 power-law diapir (pure shear, relaxation, cutoffs) on the GPU
+    and in the oracle, advanced by the tests below in two halves of 1000
+    iterations (each half is one test, so the run reports between them)."""
+    st = setups.diapir_sector(255, n_exp=3.0, bc_mode=BC_PURE_SHEAR, sh=1.0, relax=0.5, eta_min=1e-4, eta_max=1e4)
+    run = {"st": st, "o": Oracle(st), "s": _solver(st), "done": 0}
+    yield run
+    run["s"].close()
+
+
+def _config2_advance(run, upto, chunk=100):
+    st, o, s = run["st"], run["o"], run["s"]
     t0 = time.time()
-    for done in range(chunk, iters + 1, chunk):
+    while run["done"] < upto:
         s.pt_iterate(st.params, chunk)
         o.step(chunk)
-        say(f"  {done} iterations, {time.time() - t0:.0f} s")
+        run["done"] += chunk
+        say(f"  {run['done']} iterations, {time.time() - t0:.0f} s")
+    worst = _compare_all(s, o)
+    say(f"worst rel-L2 over all fields after {run['done']} iterations", worst)
+
+
+def test_config2_255_powerlaw_pure_shear_iterations_1_1000(config2_run):
+    say("255^3 power-law diapir, pure shear: fused iterations 1-1000 vs the oracle")
+    _config2_advance(config2_run, 1000)
+
+
+def test_config2_255_powerlaw_pure_shear_iterations_1001_2000(config2_run):
+    """Iterations 1001-2000 from the state the test above left (all 2000 when
+    run alone); the power law is active in every interior cell then."""
+    say("255^3 power-law diapir, pure shear: fused iterations 1001-2000 vs the oracle")
+    _config2_advance(config2_run, 2000)
+    o = config2_run["o"]
     eta, ini, pl = o.f["eta"], o.f["eta_ini"], o.f["eta_PL"]
     inner = (slice(1, -1),) * 3
     fin = float(np.isfinite(pl).mean())
@@ -146,9 +169,6 @@ def test_config2_255_powerlaw_pure_shear_2000_iterations():
     say(f"power law: eta_PL finite in {fin:.4f} of the cells, eta != 2 eta_ini in {act:.4f} of the interior; "
         f"eta in [{eta.min():.3e}, {eta.max():.3e}]")
     assert fin == 1.0 and act > 0.99
-    worst = _compare_all(s, o)
-    say("worst rel-L2 over all fields", worst)
-    s.close()
 
 
 # ---------------------------------------------------------------- 511^3 plateau
