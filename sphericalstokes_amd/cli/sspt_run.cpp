@@ -5,6 +5,11 @@
 // the reference exit test (ss_pt_solve) and prints the script's performance
 // summary (:445-453): total iterations, wall time of iterations 11..ittot and
 // T_eff = A_eff / t_it with A_eff = 26 n^3 8 B.
+// With --gradient (or --mixed-adjoint) the converged solve is followed by the
+// adjoint solve for J = the mean VR in the inclusion (ss_adjoint_solve or
+// ss_adjoint_solve_mixed, no counterpart in the reference script); J, the
+// adjoint's iteration count and its time against the forward solve's are
+// printed.
 //
 // Setups (the formulas of sphericalstokes_amd/setups.py: diapir, diapir_sector):
 //   --setup sector     r in [1,2], th in [pi/3, 2pi/3], ph in [-pi/6, pi/6],
@@ -19,6 +24,7 @@
 // counts differ there, the converged state does not).
 #include "sspt.h"
 
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -42,7 +48,8 @@ struct Opts {
     std::string setup = "sector";
     double n_exp = -1.0, vr = -1.0, sh = -1.0, eps = 1e-8;
     int maxiter = 100000, nout = 100, bc = -1, timed = 0, warmup = 10;
-    bool json = false;
+    bool json = false, gradient = false, mixed_adjoint = false;
+    double adj_tol = 1e-12;
     std::string perf_file;
 };
 
@@ -52,7 +59,10 @@ void usage()
         "usage: sspt_run [--n N | --nr N --nth N --nph N] [--setup sector|reference] [--nexp X] [--vr X]\n"
         "                [--bc free_slip|shear|pure_shear] [--sh X] [--maxiter M] [--nout K] [--eps E]\n"
         "                [--timed K [--warmup W]] [--device D] [--json] [--perf-file PATH]\n"
+        "                [--gradient | --mixed-adjoint] [--adj-tol T]\n"
         "  --timed K: W untimed, then K fused iterations with HIP-event timing instead of the solve\n"
+        "  --gradient: after a converged solve, the adjoint solve (ss_adjoint_solve) for J = mean VR in the\n"
+        "              inclusion; --mixed-adjoint: the same with FP32 sweeps (ss_adjoint_solve_mixed)\n"
         "  --bc pure_shear starts from rest; shear from the far-field simple shear\n"
         "  --perf-file: append the reference's out_perf record (nr nth nph dmp ittot wtime A_eff t_it T_eff)\n");
 }
@@ -65,6 +75,8 @@ bool parse(int argc, char **argv, Opts &o)
         const char *v = nullptr;
         if (a == "--help" || a == "-h") { usage(); std::exit(0); }
         else if (a == "--json") o.json = true;
+        else if (a == "--gradient") o.gradient = true;
+        else if (a == "--mixed-adjoint") o.gradient = o.mixed_adjoint = true;
         else if (!(v = val())) { std::fprintf(stderr, "sspt_run: %s needs a value\n", a.c_str()); return false; }
         else if (a == "--n") o.nr = o.nth = o.nph = std::atoi(v);
         else if (a == "--nr") o.nr = std::atoi(v);
@@ -75,6 +87,7 @@ bool parse(int argc, char **argv, Opts &o)
         else if (a == "--vr") o.vr = std::atof(v);
         else if (a == "--sh") o.sh = std::atof(v);
         else if (a == "--eps") o.eps = std::atof(v);
+        else if (a == "--adj-tol") o.adj_tol = std::atof(v);
         else if (a == "--maxiter") o.maxiter = std::atoi(v);
         else if (a == "--nout") o.nout = std::atoi(v);
         else if (a == "--timed") o.timed = std::atoi(v);
@@ -186,10 +199,45 @@ int main(int argc, char **argv)
             std::fclose(f);
         }
     }
+    // the gradient of J = mean VR in the inclusion (the adjoint solve at the converged state)
+    std::string adj_json;
+    int arc = SS_OK;
+    if (o.gradient && rc == SS_OK) {
+        double J = 0.0, aerr = 0.0, asec = 0.0;
+        int aconv = 0;
+        char buf[256];
+        if (o.mixed_adjoint) {
+            ss_mixed_opts mo{20, 200000, o.nout, 1e-4, o.adj_tol};
+            ss_adjoint_mixed_result ar;
+            arc = ss_adjoint_solve_mixed(c, &p, r, radius, &mo, &ar);
+            if (arc != SS_OK && arc != SS_NOT_CONVERGED) return fail(c, arc, "ss_adjoint_solve_mixed");
+            J = ar.J, aerr = ar.err, asec = ar.seconds, aconv = ar.converged;
+            std::printf("Adjoint (mixed): %d outer steps, %d FP32 sweeps, err=%1.3e, time = %1.3e sec (%.2f x the "
+                        "forward solve)\n", ar.outer, ar.inner_total, aerr, asec, asec / wtime);
+            std::snprintf(buf, sizeof buf, "\"adjoint_outer\": %d, \"adjoint_sweeps\": %d, ", ar.outer, ar.inner_total);
+        } else {
+            ss_adjoint_result ar;
+            // the adjoint iteration contracts as the forward one: twice the forward cap
+            const int amax = o.maxiter > INT_MAX / 2 ? INT_MAX : 2 * o.maxiter;
+            arc = ss_adjoint_solve(c, &p, r, radius, amax, o.nout, o.adj_tol, &ar);
+            if (arc != SS_OK && arc != SS_NOT_CONVERGED) return fail(c, arc, "ss_adjoint_solve");
+            J = ar.J, aerr = ar.err, asec = ar.seconds, aconv = ar.converged;
+            std::printf("Adjoint: %d iterations, err=%1.3e, time = %1.3e sec (%.2f x the forward solve)\n", ar.iters,
+                        aerr, asec, asec / wtime);
+            std::snprintf(buf, sizeof buf, "\"adjoint_iters\": %d, ", ar.iters);
+        }
+        std::printf("J = %.17g (mean VR in the inclusion)%s\n", J, aconv ? "" : " (adjoint not converged)");
+        char tail[256];
+        std::snprintf(tail, sizeof tail, "\"J\": %.17g, \"adjoint_err\": %.17g, \"adjoint_converged\": %s, "
+                      "\"adjoint_seconds\": %.6f, ", J, aerr, aconv ? "true" : "false", asec);
+        adj_json = std::string(", ") + buf + tail;
+        adj_json.resize(adj_json.size() - 2);
+    }
     if (o.json)
         std::printf("{\"n\": [%d, %d, %d], \"ittot\": %d, \"iter_end\": %d, \"err\": %.17g, \"converged\": %s, "
-                    "\"seconds\": %.6f, \"T_eff_GBps\": %.3f}\n",
-                    nr, nth, nph, res.ittot, res.iter_end, res.err, rc == SS_OK ? "true" : "false", wtime, T_eff);
+                    "\"seconds\": %.6f, \"T_eff_GBps\": %.3f%s}\n",
+                    nr, nth, nph, res.ittot, res.iter_end, res.err, rc == SS_OK ? "true" : "false", wtime, T_eff,
+                    adj_json.c_str());
     ss_destroy(c);
-    return rc == SS_OK ? 0 : 3;
+    return rc == SS_OK && arc == SS_OK ? 0 : 3;
 }

@@ -47,3 +47,25 @@ def test_cli_reference_setup_equals_python():
 def test_cli_timed_reports_t_eff():
     got, _ = _cli("--n", "63", "--timed", "20")
     assert got["iterations"] == 20 and got["iter_ms"] > 0 and got["T_eff_GBps"] > 0
+
+
+@pytest.mark.parametrize("mode", ["--gradient", "--mixed-adjoint"])
+def test_cli_gradient_equals_python(mode):
+    """--gradient / --mixed-adjoint after the sector solve at 15^3: J and the
+    adjoint's iteration counts and err equal the Python-driven solves of the
+    same setup (ss_adjoint_solve / ss_adjoint_solve_mixed through ctypes)."""
+    got, out = _cli("--n", "15", "--maxiter", "400000", "--nout", "100", "--eps", "1e-12", mode, "--adj-tol", "1e-12")
+    st = setups.diapir_sector(15)
+    with Solver(15, 15, 15) as s:
+        s.load_setup(st)
+        f = s.pt_solve(st.params, 400_000, 100, 1e-12)
+        if mode == "--gradient":
+            a = s.adjoint_solve(st.params, 1.5, 0.1 ** 2, maxiter=800_000, ncheck=100, tol=1e-12)
+            assert got["adjoint_iters"] == a["iters"]
+        else:
+            a = s.adjoint_solve_mixed(st.params, 1.5, 0.1 ** 2, max_outer=20, max_inner=200_000, nout_inner=100,
+                                      tol_inner=1e-4, tol=1e-12)
+            assert (got["adjoint_outer"], got["adjoint_sweeps"]) == (a["outer"], a["inner_total"])
+    assert got["ittot"] == f["ittot"] and got["adjoint_converged"] and a["converged"]
+    assert got["J"] == a["J"] and got["adjoint_err"] == a["err"]
+    assert "J = " in out
