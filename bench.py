@@ -140,7 +140,7 @@ def adjoint_solve_ratio(n, eps, mixed=True):
            "adjoint_iters": a["iters"], "adjoint_converged": a["converged"],
            "ratio_adjoint_over_forward": round(ta / tf, 4)}
     if mixed:
-        m = s.adjoint_solve_mixed(p, 1.5, 0.1 ** 2, nout_inner=500, tol=eps)
+        m = s.adjoint_solve_mixed(p, 1.5, 0.1 ** 2, nout_inner=100, tol=eps)
         print(f"  adjoint {n}^3: mixed adjoint {m['outer']} outer steps, {m['inner_total']} FP32 sweeps in "
               f"{m['seconds']:.2f} s", file=sys.stderr, flush=True)
         diff = max(float(np.abs(m[k] - a[k]).max() / np.abs(a[k]).max()) for k in ("grad_rho", "grad_log10_eta_ini"))
