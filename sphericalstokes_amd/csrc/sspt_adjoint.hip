@@ -748,7 +748,8 @@ __global__ void k_adj_zero_u64(unsigned long long *p, int n)
 // Increment test of the FP32 correction sweeps: max|new - old| (red[0..2])
 // and max|new| (red[3..5]) of lam_VR, lam_Vth, lam_Vph over the faces
 // compute_V! updates (both sets BC-folded), as the bit patterns of
-// non-negative doubles (ordered as integers).
+// non-negative doubles (ordered as integers); a non-finite value anywhere
+// sets red[0] to the NaN pattern above every finite one.
 // ADJ_CHECK_PLANES r-planes per thread, one set of atomics per wavefront.
 constexpr int ADJ_CHECK_PLANES = 32;
 __global__ void __launch_bounds__(256) k_adj_inc_f32(Box b, const float *__restrict__ n0, const float *__restrict__ n1,
@@ -759,6 +760,7 @@ __global__ void __launch_bounds__(256) k_adj_inc_f32(Box b, const float *__restr
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     double me[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0};
+    int bad = 0; // fmax drops NaN: non-finite values are flagged apart
     const int i0 = blockIdx.z * ADJ_CHECK_PLANES, i1 = min(i0 + ADJ_CHECK_PLANES, b.nr + 1);
     for (int i = i0; i < i1 && j <= b.nt && k <= b.np; ++i) {
         const long long o = b.off(i, j, k);
@@ -767,8 +769,9 @@ __global__ void __launch_bounds__(256) k_adj_inc_f32(Box b, const float *__restr
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             if (!in[c]) continue;
-            const double v = N[c][o];
-            me[c] = fmax(me[c], fabs(v - (double)O[c][o]));
+            const double v = N[c][o], d = fabs(v - (double)O[c][o]);
+            bad |= !isfinite(v) || !isfinite(d);
+            me[c] = fmax(me[c], d);
             mv[c] = fmax(mv[c], fabs(v));
         }
     }
@@ -778,7 +781,9 @@ __global__ void __launch_bounds__(256) k_adj_inc_f32(Box b, const float *__restr
             me[c] = fmax(me[c], __shfl_xor(me[c], m, 64));
             mv[c] = fmax(mv[c], __shfl_xor(mv[c], m, 64));
         }
+    for (int m = 32; m >= 1; m >>= 1) bad |= __shfl_xor(bad, m, 64);
     if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0) {
+        if (bad) atomicMax(&red[0], 0x7ff8000000000000ull);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             if (me[c] > 0.0) atomicMax(&red[c], (unsigned long long)__double_as_longlong(me[c]));
@@ -1023,7 +1028,7 @@ adjf::AArgs<float> f32_args(ss_ctx *c, const AdjArgs &a)
 }
 
 // max over the velocity components of red[c] / red[3 + c] (bit patterns of
-// non-negative doubles); -1 when a NaN was seen
+// non-negative doubles); -1 when a value is not finite
 double ratio_of(const unsigned long long *red)
 {
     double err = 0.0;
@@ -1031,7 +1036,7 @@ double ratio_of(const unsigned long long *red)
         double dv, vv;
         memcpy(&dv, &red[q], 8);
         memcpy(&vv, &red[3 + q], 8);
-        if (std::isnan(dv) || std::isnan(vv)) return -1.0;
+        if (!std::isfinite(dv) || !std::isfinite(vv)) return -1.0;
         if (vv > 0.0) err = std::fmax(err, dv / vv);
     }
     return err;
@@ -1247,7 +1252,7 @@ int ss_adjoint_solve_mixed(ss_ctx *c, const ss_params *p, double r, double radiu
             return afail(c, SS_ERR_STATE, "adjoint check readback");
         if ((rc = sync_status(c, "adjoint outer step")) != SS_OK) return rc;
         const double err = ratio_of(c->red_host);
-        if (err < 0.0) return afail(c, SS_ERR_NONFINITE, "adjoint NaN");
+        if (err < 0.0) return afail(c, SS_ERR_NONFINITE, "mixed adjoint: non-finite adjoint value");
         out->err = err;
         out->outer = outer;
         if (err < o->tol) { // Y is the iterate
@@ -1295,7 +1300,7 @@ int ss_adjoint_solve_mixed(ss_ctx *c, const ss_params *p, double r, double radiu
                     return afail(c, SS_ERR_STATE, "adjoint inner readback");
                 if ((rc = sync_status(c, "adjoint FP32 sweeps")) != SS_OK) return rc;
                 const double ein = ratio_of(c->red_host);
-                if (ein < 0.0) return afail(c, SS_ERR_NONFINITE, "adjoint FP32 sweep NaN");
+                if (ein < 0.0) return afail(c, SS_ERR_NONFINITE, "mixed adjoint: non-finite value in the FP32 sweeps");
                 if (ein < tin) break;
             }
         }

@@ -124,3 +124,20 @@ def test_autograd_function_mixed_adjoint():
         grads[mixed] = (rho.grad.numpy(), le.grad.numpy())
     for a, b in zip(grads[True], grads[False]):
         assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+
+
+def test_mixed_adjoint_nonfinite_state_is_an_error():
+    """A NaN in the converged state (here one viscosity) reaches the adjoint
+    fields in the first FP64 step; the solve stops there with
+    SS_ERR_NONFINITE instead of sweeping on."""
+    from sphericalstokes_amd._lib import SS_ERR_NONFINITE, SSError
+    st = setups.diapir_sector(15, n_exp=1.0, bc_mode=BC_FREE_SLIP, vr=10.0)
+    with _solver(st) as s:
+        f = s.pt_solve(st.params, 2_000_000, 100, 1e-10)
+        assert f["status"] == "SS_OK", f
+        eta = s.download("eta")
+        eta[7, 7, 7] = np.nan
+        s.upload("eta", eta)
+        with pytest.raises(SSError) as e:
+            s.adjoint_solve_mixed(st.params, R0, RAD, tol=1e-13)
+        assert e.value.code == SS_ERR_NONFINITE
