@@ -249,6 +249,8 @@ int ss_adjoint_solve(ss_ctx *ctx, const ss_params *p, double r, double radius, i
  * or o->max_inner sweeps, then mu += F d in FP64.  The inner tolerance is
  * o->tol_inner, loosened (to at most 1e-2) when the reduction still needed
  * is smaller than the last outer step's.  Gradients as ss_adjoint_solve.
+ * Returns SS_OK, SS_NOT_CONVERGED (o->max_outer steps) or SS_ERR_NONFINITE
+ * (a non-finite adjoint value at an FP64 step or an FP32 check).
  * Allocates 49 FP32 boxes on first use. */
 typedef struct ss_adjoint_mixed_result {
     int outer, inner_total, converged;
